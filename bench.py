@@ -1,0 +1,194 @@
+"""Benchmark of the per-step PM loop on one MI355X.
+
+Workload (default): 1024^3 particles on a 1024^3 grid, periodic box 2000 Mpc/h,
+Zel'dovich initial conditions at z = 50 (Eisenstein-Hu no-wiggle P(k), sigma_8 = 0.81,
+Philox white noise, seed 2), 50 a-uniform steps to z = 0; FP32 engine (int32 cell +
+FP32 offset positions, int64 fixed-point deposit, complex64 FFT).  A step is one pass of
+drift -> CIC deposit -> FFT Poisson/gradient solve -> gather/kick -> drift over every
+particle, with inputs resident in HBM.  W warmup steps, then K timed steps bracketed by
+device synchronisation; the value is particle-steps per second.
+
+Also reported on the same line:
+  roofline     : the dominant kernel (largest share of the timed region, timed with HIP
+                 events on the engine stream) against MI355X HBM bandwidth, from its
+                 algorithmic bytes per launch;
+  fp64_plain   : the plain FP64 GPU path (reference-structured kernels in FP64) on the
+                 same workload, and the FP32/FP64 speed-up;
+  cpu_baseline : the reference algorithm restated on the host CPU (the oracle in the
+                 reference's FP32 rounding) on 64^3 particles x 60 steps, threads stated.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s, 1024^3 PM FP32 vs FP64 at stated P(k) err; % HBM roofline"
+HBM_PEAK = 8.0e12
+
+
+def kernel_bytes(name, Np, G):
+    """Algorithmic bytes per launch of each timed kernel region: what the kernel must
+    read and write once (no re-reads, no atomics beyond one write per cell)."""
+    table = {
+        "drift": 60 * Np,                      # read pos 24 + vel 12, write pos 24
+        "zero_density": 8 * G,                 # write int64 density
+        "deposit": 24 * Np + 8 * G,            # read pos, write density
+        "fft_fwd_z_density": 16 * G,           # read int64, write complex64
+        "fft_fwd_y": 16 * G,                   # read + write complex64
+        "fft_x_solve": 24 * G,                 # read rho^, write P and Q
+        "fft_inv_y": 32 * G,                   # two grids, read + write each
+        "fft_inv_z_fxy": 16 * G,               # read + write (Fx, Fy)
+        "fft_inv_z_fz": 12 * G,                # read complex, write Fz
+        "gather_kick_drift": 72 * Np + 12 * G,  # pos+vel read and write, force read
+        "gather_kick": 48 * Np + 12 * G,
+    }
+    return table.get(name)
+
+
+def make_ics(hg, p, ts):
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "eh.ipk")
+        hg.write_ipk_eh(p, fn, 100.0, 200001)
+        hdr, vals = hg.read_ipk(fn)
+    return hdr, vals
+
+
+def run_engine(hg, p, steps, warmup, timing=True):
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    hdr, vals = make_ics(hg, p, ts)
+    n_total = warmup + steps
+    sched = hg.schedule(p, n_total, merge=True)
+    with hg.MemoryManager(p) as m:
+        t0 = time.perf_counter()
+        hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
+        m.synchronize()
+        t_ic = time.perf_counter() - t0
+        for s in range(warmup):
+            m.step(sched[s])
+        m.synchronize()
+        if timing:
+            m.set_timing(True)
+            m.reset_timings()
+        m.synchronize()
+        t0 = time.perf_counter()
+        for s in range(warmup, n_total):
+            m.step(sched[s])
+        m.synchronize()
+        el = time.perf_counter() - t0
+        tim = m.timings() if timing else {}
+        dev_bytes = m.device_bytes()
+    return el, tim, t_ic, dev_bytes
+
+
+def cpu_baseline(hg, threads):
+    """Reference algorithm on the host: the oracle in the reference's FP32 rounding
+    (float3 positions, FP32 CIC, reference calcGreens), 64^3 particles x 60 steps."""
+    from oracle import oracle as orc
+    orc.set_threads(threads)
+    ng, n = 64, 60
+    p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001)
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    hdr, vals = make_ics(hg, p, ts)
+    a = 1 / (1 + p.z_ini)
+    D, _ = hg.growth(p, a)
+    _, dD = hg.growth(p, a - ts.deltaT / 2)
+    pos, vel = orc.ic_f64(ng, ng, 128.0, 21082001, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
+    co = hg.schedule_array(p, n, merge=False)
+    t0 = time.perf_counter()
+    _, _, t_loop = orc.run_ref32(ng, ng, co, pos.astype(np.float32), vel.astype(np.float32))
+    el = time.perf_counter() - t0
+    return {"value": ng ** 3 * n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"64^3 particles, 64^3 grid, box 128 Mpc/h, z=200->0, {n} steps (reference FP32 "
+                      f"rounding, FP64 FFT), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ng", type=int, default=1024)
+    ap.add_argument("--rl", type=float, default=2000.0)
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs a single GPU (one rank); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    import hacc_gpm_amd as hg
+
+    ng = args.ng
+    p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221)
+    Np = ng ** 3
+    G = ng ** 3
+    el, tim, t_ic, dev_bytes = run_engine(hg, p, args.steps, args.warmup)
+    value = Np * args.steps / el
+    ms_step = el * 1e3 / args.steps
+    # dominant kernel region
+    best = max(tim.items(), key=lambda kv: kv[1][0])
+    name, (ms_tot, launches) = best
+    per_launch = kernel_bytes(name, Np, G)
+    # inv_y region holds two launches: bytes given per region instance
+    avg_ms = ms_tot / max(launches, 1)
+    achieved = per_launch / (avg_ms * 1e-3) if per_launch else None
+    breakdown = {k: {"ms_per_step": v[0] / args.steps, "launches": v[1],
+                     "GBps": (kernel_bytes(k, Np, G) / (v[0] / v[1] * 1e-3) / 1e9) if kernel_bytes(k, Np, G) and v[1] else None}
+                 for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0])}
+    step_alg = 96 * Np + 136 * G  # sum of the per-kernel algorithmic bytes of one fused step
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Zel'dovich ICs at z=50 from Philox white noise and an Eisenstein-Hu no-wiggle P(k)",
+        "config": {"workload": f"pm_step_{ng}^3_particles_{ng}^3_grid_box{int(args.rl)}", "np": ng, "ng": ng,
+                   "rl": args.rl, "z_ini": 50, "steps_in_run": 50, "positions": "int32 cell + f32 offset",
+                   "deposit": "int64 fixed point 2^-24"},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "alg_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
+        "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (ms_step * 1e-3) / 1e9,
+                          "frac": step_alg / (ms_step * 1e-3) / HBM_PEAK},
+        "kernels": breakdown,
+        "ic_seconds": t_ic,
+        "device_GB": dev_bytes / 1e9,
+    }
+    if not args.no_fp64:
+        p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221, precision=hg.FP64)
+        try:
+            el64, _, _, b64 = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
+            v64 = Np * args.fp64_steps / el64
+            out["fp64_plain"] = {"value": v64, "ms_per_step": el64 * 1e3 / args.fp64_steps,
+                                 "steps": args.fp64_steps, "device_GB": b64 / 1e9}
+            out["speedup_vs_fp64"] = value / v64
+        except Exception as e:  # report, do not hide
+            out["fp64_plain"] = {"error": str(e)}
+    if not args.no_cpu:
+        th = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(hg, th)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,983 @@
+// capi.cpp — the context (serial::MemoryManager equivalent) and the C-ABI entry
+// points of include/hgpm.h.  One HIP stream per context; every launch is
+// asynchronous on it and host calls synchronise only where data crosses to the host.
+#include "../../include/hgpm.h"
+#include "host.h"
+#include "internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace hgpm;
+
+namespace {
+struct KStat {
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+struct PendingEv {
+    int slot;
+    hipEvent_t a, b;
+};
+} // namespace
+
+struct hgpm_ctx {
+    hgpm_params p{};
+    int ng = 0, np = 0, prec = HGPM_FP32;
+    int64_t N = 0, G = 0, Gic = 0;
+    hipStream_t st = nullptr;
+    int device = 0;
+    std::string err;
+    size_t bytes = 0;
+    const FftOps* fft = nullptr;    // length ng
+    const FftOps* fftic = nullptr;  // length np
+    // FP32 engine
+    PosSplitH* pos = nullptr;
+    float* vel = nullptr;
+    float2* A = nullptr;             // spectrum / Fz^ (8 B per cell)
+    float2* B = nullptr;             // int64 density / Fx^+iFy^ / (Fx,Fy) (8 B per cell)
+    float* fz = nullptr;             // Fz (4 B per cell)
+    unsigned long long* total = nullptr;
+    float2 *twf = nullptr, *twr = nullptr, *twf_ic = nullptr;
+    float *sk = nullptr, *hs = nullptr;
+    // FP64 plain path
+    double *pos64 = nullptr, *vel64 = nullptr, *grad64 = nullptr;
+    double2* grid64 = nullptr;
+    double2 *tw64 = nullptr, *tw64_ic = nullptr;
+    // P(k)
+    double *pk_partial = nullptr, *pk_sums = nullptr, *pk_win = nullptr;
+    long long* pk_counts = nullptr;
+    int pk_blocks = 0;
+    int* flag = nullptr;
+    // state
+    bool have_particles = false, have_density = false, have_force = false;
+    // per-kernel timing
+    bool timing = false;
+    std::vector<std::string> names;
+    std::map<std::string, int> slot_of;
+    std::vector<KStat> stats;
+    std::vector<PendingEv> pending;
+    std::vector<hipEvent_t> evpool;
+    // tunables (reserved for the autotuner)
+    std::map<std::string, int> tun;
+};
+
+namespace {
+int fail(hgpm_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    else set_global_error(m);
+    return code;
+}
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, HGPM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define LAUNCHCHK(ctx)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = hipGetLastError();                                                      \
+        if (e_ != hipSuccess) return fail(ctx, HGPM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T> int dalloc(hgpm_ctx* c, T** p, size_t bytes, const char* what) {
+    hipError_t e = hipMalloc((void**)p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, HGPM_ERR_ALLOC, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
+                                           " B) failed: " + hipGetErrorString(e));
+    }
+    c->bytes += bytes;
+    return HGPM_OK;
+}
+
+hipEvent_t ev_get(hgpm_ctx* c) {
+    if (!c->evpool.empty()) {
+        hipEvent_t e = c->evpool.back();
+        c->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+int slot(hgpm_ctx* c, const char* name) {
+    auto it = c->slot_of.find(name);
+    if (it != c->slot_of.end()) return it->second;
+    int s = (int)c->names.size();
+    c->names.push_back(name);
+    c->stats.push_back(KStat{});
+    c->slot_of[name] = s;
+    return s;
+}
+void fold_pending(hgpm_ctx* c) {
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->st);
+    for (auto& pe : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+            c->stats[pe.slot].ms += ms;
+            c->stats[pe.slot].launches += 1;
+        }
+        c->evpool.push_back(pe.a);
+        c->evpool.push_back(pe.b);
+    }
+    c->pending.clear();
+}
+// Brackets the launches of its scope with HIP events on the context stream.
+struct Timed {
+    hgpm_ctx* c;
+    int s = -1;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(hgpm_ctx* c_, const char* name) : c(c_) {
+        if (!c->timing) return;
+        if (c->pending.size() > 4096) fold_pending(c);
+        s = slot(c, name);
+        a = ev_get(c);
+        b = ev_get(c);
+        if (a) (void)hipEventRecord(a, c->st);
+    }
+    ~Timed() {
+        if (s < 0 || !a || !b) return;
+        (void)hipEventRecord(b, c->st);
+        c->pending.push_back(PendingEv{s, a, b});
+    }
+};
+
+bool supported_n(int n) { return fft_ops(n) != nullptr; }
+
+// Host twiddle tables (double) converted for upload.
+template <typename V>
+int upload_twiddles(hgpm_ctx* c, const FftOps* ops, V** f, V** r) {
+    int n = ops->n, S = ops->stages;
+    std::vector<double> tf(2 * (size_t)S * n), tr(2 * (size_t)S * n);
+    ops->twiddles(tf.data(), tr.data());
+    std::vector<V> vf((size_t)S * n), vr((size_t)S * n);
+    typedef decltype(V().x) T;
+    for (size_t i = 0; i < vf.size(); i++) {
+        vf[i].x = (T)tf[2 * i]; vf[i].y = (T)tf[2 * i + 1];
+        vr[i].x = (T)tr[2 * i]; vr[i].y = (T)tr[2 * i + 1];
+    }
+    int rc = dalloc(c, f, sizeof(V) * vf.size(), "twiddles");
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(*f, vf.data(), sizeof(V) * vf.size(), hipMemcpyHostToDevice));
+    if (r) {
+        rc = dalloc(c, r, sizeof(V) * vr.size(), "twiddles");
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpy(*r, vr.data(), sizeof(V) * vr.size(), hipMemcpyHostToDevice));
+    }
+    return HGPM_OK;
+}
+
+// 3-D transform of an n^3 grid through the generic passes (z, y, x).
+int fft3d_f32(hgpm_ctx* c, const FftOps* ops, float2* d, const float2* tw, int dir) {
+    Timed t(c, "fft3d_f32");
+    ops->pass_f32(d, 2, dir, tw, c->st);
+    ops->pass_f32(d, 1, dir, tw, c->st);
+    ops->pass_f32(d, 0, dir, tw, c->st);
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+int fft3d_f64(hgpm_ctx* c, const FftOps* ops, double2* d, const double2* tw, int dir) {
+    Timed t(c, "fft3d_f64");
+    ops->pass_f64(d, 2, dir, tw, c->st);
+    ops->pass_f64(d, 1, dir, tw, c->st);
+    ops->pass_f64(d, 0, dir, tw, c->st);
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+float ref_mass(const hgpm_ctx* c) {
+    float gp = (float)c->ng / (float)c->np;
+    return gp * gp * gp;
+}
+
+// ------------------------------------------------------------ FP32 ops ----
+int f32_deposit(hgpm_ctx* c) {
+    {
+        Timed t(c, "zero_density");
+        launch_zero_u64((unsigned long long*)c->B, (size_t)c->G, c->st);
+        launch_zero_u64(c->total, 1, c->st);
+    }
+    Timed t(c, "deposit");
+    launch_deposit(c->pos, c->N, c->ng, ref_mass(c), (long long*)c->B, c->total, c->st);
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+int f32_solve(hgpm_ctx* c) {
+    const FftOps* f = c->fft;
+    {
+        Timed t(c, "fft_fwd_z_density");
+        f->fwd_z_density((const long long*)c->B, c->total, c->A, c->twf, c->st);
+    }
+    {
+        Timed t(c, "fft_fwd_y");
+        f->pass_f32(c->A, 1, -1, c->twf, c->st);
+    }
+    {
+        Timed t(c, "fft_x_solve");
+        f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, c->st);
+    }
+    {
+        Timed t(c, "fft_inv_y");
+        f->pass_f32(c->B, 1, +1, c->twf, c->st);
+        f->pass_f32(c->A, 1, +1, c->twf, c->st);
+    }
+    {
+        Timed t(c, "fft_inv_z_fxy");
+        f->pass_f32(c->B, 2, +1, c->twf, c->st);
+    }
+    {
+        Timed t(c, "fft_inv_z_fz");
+        f->inv_z_real(c->A, c->fz, c->twf, c->st);
+    }
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+// ------------------------------------------------------------ FP64 ops ----
+int f64_deposit(hgpm_ctx* c, const double* pos) {
+    Timed t(c, "f64_cic");
+    launch_f64_cic(pos, c->N, c->ng, pow((double)c->ng / (double)c->np, 3.0), c->grid64, c->st);
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+int f64_solve(hgpm_ctx* c) {
+    int rc = fft3d_f64(c, c->fft, c->grid64, c->tw64, -1);
+    if (rc) return rc;
+    {
+        Timed t(c, "f64_copy_to_grad");
+        launch_f64_copy_to_grad(c->grid64, c->grad64, c->G, c->st);
+    }
+    for (int d = 0; d < 3; d++) {
+        {
+            Timed t(c, "f64_kspace");
+            launch_f64_kspace(c->grid64, c->grad64, c->ng, d, c->st);
+        }
+        rc = fft3d_f64(c, c->fft, c->grid64, c->tw64, +1);
+        if (rc) return rc;
+        {
+            Timed t(c, "f64_real_to_grad");
+            launch_f64_real_to_grad(c->grid64, c->grad64, c->G, d, c->st);
+        }
+    }
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+int need_particles(hgpm_ctx* c) {
+    if (!c->have_particles) return fail(c, HGPM_ERR_STATE, "no particles: call hgpm_set_particles* or hgpm_generate_ic");
+    return HGPM_OK;
+}
+
+// P(k) bin layout (power.cu:174-187)
+void pk_layout(const hgpm_ctx* c, double* minK, double* binDelta, double* kbins) {
+    const int ng = c->ng, nb = c->p.pk_bins;
+    double new_rl = c->p.rl / pow(2, c->p.pk_folds);
+    double d = (2 * HGPM_PI_D) / new_rl;
+    double mk = d;
+    double maxK = sqrt(3 * ((double)((ng / 2) * (ng / 2)) * d * d));
+    double bd = (maxK - mk) / ((double)nb);
+    for (int i = 0; i < nb; i++) kbins[i] = mk + bd * (double)(((float)i) + 0.5f);
+    *minK = mk;
+    *binDelta = bd;
+}
+
+// spectrum already in A (FP32) or grid64 (FP64): bin it
+int pk_bin_spectrum(hgpm_ctx* c, int fold, double* k, double* pk, int64_t* count) {
+    const int nb = c->p.pk_bins;
+    std::vector<double> kb(nb);
+    double minK, bd;
+    pk_layout(c, &minK, &bd, kb.data());
+    const double rl_fold = c->p.rl / pow(2, fold);
+    const double scale = (1.0 / ((double)c->ng * c->ng * c->ng)) * (c->p.rl / (double)c->ng);
+    {
+        Timed t(c, "power_bins");
+        if (c->prec == HGPM_FP32)
+            launch_power_bins<float2>(c->A, c->ng, rl_fold, minK, bd, nb, c->pk_win, scale, c->pk_partial,
+                                      c->pk_counts, c->pk_blocks, c->pk_sums, c->st);
+        else
+            launch_power_bins<double2>(c->grid64, c->ng, rl_fold, minK, bd, nb, c->pk_win, scale, c->pk_partial,
+                                       c->pk_counts, c->pk_blocks, c->pk_sums, c->st);
+        LAUNCHCHK(c);
+    }
+    std::vector<double> sums(nb);
+    std::vector<long long> cnt(nb);
+    HIPCHK(c, hipMemcpyAsync(sums.data(), c->pk_sums, sizeof(double) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->pk_counts, sizeof(long long) * nb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int i = 0; i < nb; i++) {
+        if (k) k[i] = kb[i];
+        if (count) count[i] = cnt[i];
+        if (pk) pk[i] = cnt[i] ? sums[i] / (double)cnt[i] : 0.0;
+    }
+    return HGPM_OK;
+}
+} // namespace
+
+extern "C" {
+
+const char* hgpm_version(void) { return "hgpm 0.1.0 (gfx950)"; }
+
+const char* hgpm_last_error(const hgpm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
+
+int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
+    if (!p || !out) return fail(nullptr, HGPM_ERR_ARG, "hgpm_create: null argument");
+    *out = nullptr;
+    if (!supported_n(p->ng) || !supported_n(p->np))
+        return fail(nullptr, HGPM_ERR_ARG, "unsupported NG/NP " + std::to_string(p->ng) + "/" + std::to_string(p->np) +
+                                               " (supported: 32 64 96 128 192 256 384 512 768 1024)");
+    if (p->np > p->ng) return fail(nullptr, HGPM_ERR_UNSUPPORTED, "NP > NG is not supported");
+    if (p->pk_bins < 1 || p->pk_bins > 256) return fail(nullptr, HGPM_ERR_ARG, "PK_BINS must be in [1,256]");
+    if (p->pk_folds < 0 || p->pk_folds > 8) return fail(nullptr, HGPM_ERR_ARG, "PK_FOLDS must be in [0,8]");
+    if (!(p->rl > 0)) return fail(nullptr, HGPM_ERR_ARG, "RL must be > 0");
+    if (p->precision != HGPM_FP32 && p->precision != HGPM_FP64)
+        return fail(nullptr, HGPM_ERR_ARG, "precision must be HGPM_FP32 or HGPM_FP64");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return fail(nullptr, HGPM_ERR_HIP, "no HIP device available");
+    }
+    hgpm_ctx* c = new hgpm_ctx();
+    c->p = *p;
+    c->ng = p->ng;
+    c->np = p->np;
+    c->prec = p->precision;
+    c->N = (int64_t)p->np * p->np * p->np;
+    c->G = (int64_t)p->ng * p->ng * p->ng;
+    c->Gic = c->G; // np <= ng
+    c->fft = fft_ops(p->ng);
+    c->fftic = fft_ops(p->np);
+    (void)hipGetDevice(&c->device);
+    int rc = HGPM_OK;
+    auto bail = [&](int code) {
+        std::string m = c->err;
+        hgpm_destroy(c);
+        set_global_error(m);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(HGPM_ERR_HIP);
+    }
+    const size_t N = (size_t)c->N, G = (size_t)c->G;
+    if (c->prec == HGPM_FP32) {
+        if ((rc = dalloc(c, &c->pos, sizeof(PosSplitH) * N, "pos")) ||
+            (rc = dalloc(c, &c->vel, sizeof(float) * 3 * N, "vel")) ||
+            (rc = dalloc(c, &c->A, sizeof(float2) * G, "grid A")) ||
+            (rc = dalloc(c, &c->B, sizeof(float2) * G, "grid B")) ||
+            (rc = dalloc(c, &c->fz, sizeof(float) * G, "Fz")) ||
+            (rc = dalloc(c, &c->total, sizeof(unsigned long long), "total")))
+            return bail(rc);
+        if ((rc = upload_twiddles<float2>(c, c->fft, &c->twf, &c->twr))) return bail(rc);
+        if (c->np != c->ng) {
+            if ((rc = upload_twiddles<float2>(c, c->fftic, &c->twf_ic, nullptr))) return bail(rc);
+        } else c->twf_ic = c->twf;
+        // per-axis solve tables: sin(2 pi m_s/ng) (signed mode) and sin^2(pi m/ng)
+        std::vector<float> sk(p->ng), hs(p->ng);
+        for (int m = 0; m < p->ng; m++) {
+            int ms = m > p->ng / 2 - 1 ? m - p->ng : m;
+            sk[m] = (float)sin(2.0 * HGPM_PI_D * ms / p->ng);
+            double h = sin(HGPM_PI_D * m / p->ng);
+            hs[m] = (float)(h * h);
+        }
+        if ((rc = dalloc(c, &c->sk, sizeof(float) * p->ng, "sk")) || (rc = dalloc(c, &c->hs, sizeof(float) * p->ng, "hs")))
+            return bail(rc);
+        if (hipMemcpy(c->sk, sk.data(), sizeof(float) * p->ng, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->hs, hs.data(), sizeof(float) * p->ng, hipMemcpyHostToDevice) != hipSuccess) {
+            c->err = "table upload failed";
+            return bail(HGPM_ERR_HIP);
+        }
+    } else {
+        size_t grad_bytes = sizeof(double) * 4 * G;
+        if (grad_bytes < sizeof(double) * 3 * N) grad_bytes = sizeof(double) * 3 * N;
+        if ((rc = dalloc(c, &c->pos64, sizeof(double) * 3 * N, "pos64")) ||
+            (rc = dalloc(c, &c->vel64, sizeof(double) * 3 * N, "vel64")) ||
+            (rc = dalloc(c, &c->grid64, sizeof(double2) * G, "grid64")) ||
+            (rc = dalloc(c, &c->grad64, grad_bytes, "grad64")))
+            return bail(rc);
+        if ((rc = upload_twiddles<double2>(c, c->fft, &c->tw64, nullptr))) return bail(rc);
+        if (c->np != c->ng) {
+            if ((rc = upload_twiddles<double2>(c, c->fftic, &c->tw64_ic, nullptr))) return bail(rc);
+        } else c->tw64_ic = c->tw64;
+    }
+    // P(k) scratch and CIC window table (x/sin x)^4, x = pi m_s/ng
+    c->pk_blocks = power_partial_blocks(p->ng);
+    std::vector<double> win(p->ng);
+    for (int m = 0; m < p->ng; m++) {
+        int ms = m > p->ng / 2 - 1 ? m - p->ng : m;
+        if (ms == 0) win[m] = 1.0;
+        else {
+            double x = HGPM_PI_D * ms / p->ng, r = x / sin(x);
+            win[m] = r * r * r * r;
+        }
+    }
+    if ((rc = dalloc(c, &c->pk_partial, sizeof(double) * (size_t)c->pk_blocks * p->pk_bins, "pk partial")) ||
+        (rc = dalloc(c, &c->pk_sums, sizeof(double) * p->pk_bins, "pk sums")) ||
+        (rc = dalloc(c, &c->pk_counts, sizeof(long long) * p->pk_bins, "pk counts")) ||
+        (rc = dalloc(c, &c->pk_win, sizeof(double) * p->ng, "pk window")) ||
+        (rc = dalloc(c, &c->flag, sizeof(int), "flag")))
+        return bail(rc);
+    if (hipMemcpy(c->pk_win, win.data(), sizeof(double) * p->ng, hipMemcpyHostToDevice) != hipSuccess) {
+        c->err = "window upload failed";
+        return bail(HGPM_ERR_HIP);
+    }
+    *out = c;
+    return HGPM_OK;
+}
+
+int hgpm_destroy(hgpm_ctx* c) {
+    if (!c) return HGPM_OK;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->total, c->twf, c->twr, c->sk, c->hs,
+                    c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
+                    c->pk_counts, c->pk_win, c->flag};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c->twf_ic && c->twf_ic != c->twf) (void)hipFree(c->twf_ic);
+    if (c->tw64_ic && c->tw64_ic != c->tw64) (void)hipFree(c->tw64_ic);
+    for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return HGPM_OK;
+}
+
+int hgpm_device_bytes(const hgpm_ctx* c, size_t* bytes) {
+    if (!c || !bytes) return HGPM_ERR_ARG;
+    *bytes = c->bytes;
+    return HGPM_OK;
+}
+
+int hgpm_synchronize(hgpm_ctx* c) {
+    if (!c) return HGPM_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HGPM_OK;
+}
+
+// ------------------------------------------------------------ particles ----
+int hgpm_set_particles(hgpm_ctx* c, const float* pos, const float* vel) {
+    if (!c || !pos || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles: null argument");
+    const size_t n3 = 3 * (size_t)c->N;
+    if (c->prec == HGPM_FP32) {
+        float* tmp = nullptr;
+        int rc = dalloc(c, &tmp, sizeof(float) * n3, "staging");
+        if (rc) return rc;
+        c->bytes -= sizeof(float) * n3;
+        HIPCHK(c, hipMemcpyAsync(tmp, pos, sizeof(float) * n3, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->vel, vel, sizeof(float) * n3, hipMemcpyHostToDevice, c->st));
+        launch_split_from_float3(tmp, c->pos, c->N, c->ng, c->st);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        (void)hipFree(tmp);
+    } else {
+        std::vector<double> p(n3), v(n3);
+        for (size_t i = 0; i < n3; i++) {
+            double x = fmod((double)pos[i], (double)c->ng);
+            if (x < 0) x += c->ng;
+            if (x >= c->ng) x -= c->ng;
+            if (!std::isfinite(x) || !std::isfinite(vel[i]))
+                return fail(c, HGPM_ERR_ARG, "non-finite particle at index " + std::to_string(i / 3));
+            p[i] = x;
+            v[i] = vel[i];
+        }
+        HIPCHK(c, hipMemcpyAsync(c->pos64, p.data(), sizeof(double) * n3, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->vel64, v.data(), sizeof(double) * n3, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    c->have_particles = true;
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_get_particles(hgpm_ctx* c, float* pos, float* vel) {
+    if (!c || !pos || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_get_particles: null argument");
+    int rc = need_particles(c);
+    if (rc) return rc;
+    const size_t n3 = 3 * (size_t)c->N;
+    if (c->prec == HGPM_FP32) {
+        float* tmp = nullptr;
+        rc = dalloc(c, &tmp, sizeof(float) * n3, "staging");
+        if (rc) return rc;
+        c->bytes -= sizeof(float) * n3;
+        launch_split_to_float3(c->pos, tmp, c->N, c->st);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(pos, tmp, sizeof(float) * n3, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(vel, c->vel, sizeof(float) * n3, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        (void)hipFree(tmp);
+    } else {
+        std::vector<double> p(n3), v(n3);
+        HIPCHK(c, hipMemcpyAsync(p.data(), c->pos64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(v.data(), c->vel64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (size_t i = 0; i < n3; i++) { pos[i] = (float)p[i]; vel[i] = (float)v[i]; }
+    }
+    return HGPM_OK;
+}
+
+int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off, const float* vel) {
+    if (!c || !cell || !off || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles_split: null argument");
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "split layout needs an FP32 context");
+    const size_t n = (size_t)c->N;
+    std::vector<PosSplitH> h(n);
+    for (size_t i = 0; i < n; i++) {
+        for (int d = 0; d < 3; d++) {
+            int cc = cell[3 * i + d];
+            float f = off[3 * i + d];
+            if (cc < 0 || cc >= c->ng || !(f >= 0.0f && f < 1.0f))
+                return fail(c, HGPM_ERR_ARG, "split particle out of range at index " + std::to_string(i));
+        }
+        h[i] = PosSplitH{cell[3 * i], cell[3 * i + 1], cell[3 * i + 2], off[3 * i], off[3 * i + 1], off[3 * i + 2]};
+    }
+    HIPCHK(c, hipMemcpyAsync(c->pos, h.data(), sizeof(PosSplitH) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vel, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_particles = true;
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_get_particles_split(hgpm_ctx* c, int32_t* cell, float* off, float* vel) {
+    if (!c || !cell || !off || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_get_particles_split: null argument");
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "split layout needs an FP32 context");
+    int rc = need_particles(c);
+    if (rc) return rc;
+    const size_t n = (size_t)c->N;
+    std::vector<PosSplitH> h(n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->pos, sizeof(PosSplitH) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vel, c->vel, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < n; i++) {
+        cell[3 * i] = h[i].cx; cell[3 * i + 1] = h[i].cy; cell[3 * i + 2] = h[i].cz;
+        off[3 * i] = h[i].fx; off[3 * i + 1] = h[i].fy; off[3 * i + 2] = h[i].fz;
+    }
+    return HGPM_OK;
+}
+
+int hgpm_set_particles_f64(hgpm_ctx* c, const double* pos, const double* vel) {
+    if (!c || !pos || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles_f64: null argument");
+    if (c->prec != HGPM_FP64) return fail(c, HGPM_ERR_UNSUPPORTED, "FP64 layout needs an FP64 context");
+    const size_t n3 = 3 * (size_t)c->N;
+    for (size_t i = 0; i < n3; i++)
+        if (!(pos[i] >= 0.0 && pos[i] < (double)c->ng) || !std::isfinite(vel[i]))
+            return fail(c, HGPM_ERR_ARG, "FP64 particle out of [0, ng) or non-finite at index " + std::to_string(i / 3));
+    HIPCHK(c, hipMemcpyAsync(c->pos64, pos, sizeof(double) * n3, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->vel64, vel, sizeof(double) * n3, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_particles = true;
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_get_particles_f64(hgpm_ctx* c, double* pos, double* vel) {
+    if (!c || !pos || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_get_particles_f64: null argument");
+    if (c->prec != HGPM_FP64) return fail(c, HGPM_ERR_UNSUPPORTED, "FP64 layout needs an FP64 context");
+    int rc = need_particles(c);
+    if (rc) return rc;
+    const size_t n3 = 3 * (size_t)c->N;
+    HIPCHK(c, hipMemcpyAsync(pos, c->pos64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vel, c->vel64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HGPM_OK;
+}
+
+// -------------------------------------------------------------------- IC ----
+int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_delta, double ipk_k_max,
+                     double delta, double dot_delta, double fscal, double deltaT) {
+    if (!c || !ipk || ipk_bins < 2 || !(ipk_k_delta > 0)) return fail(c, HGPM_ERR_ARG, "hgpm_generate_ic: bad arguments");
+    const int np = c->np;
+    // launch_interpolate_pk range check (interpolation_initializer.cu:68-74), kept as is
+    double maxK = ((np / 2) * 2 * HGPM_PI_D) / c->p.rl;
+    maxK = sqrt(maxK * maxK * maxK);
+    if (maxK > ipk_k_max) return fail(c, HGPM_ERR_ARG, "input ipk only goes to " + std::to_string(ipk_k_max));
+    // the interpolation reads bin (int)(k/k_delta)+1 for |k| up to sqrt(3) pi np/rl
+    double kmax_grid = sqrt(3.0) * (np / 2) * 2 * HGPM_PI_D / c->p.rl;
+    if ((int)(kmax_grid / ipk_k_delta) + 1 >= ipk_bins) return fail(c, HGPM_ERR_ARG, "ipk table too short for this grid");
+    double* d_ipk = nullptr;
+    int rc = dalloc(c, &d_ipk, sizeof(double) * ipk_bins, "ipk");
+    if (rc) return rc;
+    c->bytes -= sizeof(double) * ipk_bins;
+    HIPCHK(c, hipMemcpyAsync(d_ipk, ipk, sizeof(double) * ipk_bins, hipMemcpyHostToDevice, c->st));
+    const double a = 1.0 / (1.0 + c->p.z_ini);
+    const uint32_t seed = (uint32_t)c->p.seed;
+    if (c->prec == HGPM_FP32) {
+        // reference float rounding of the velocity factor (place_particles_initializer.cu:200-205)
+        float velA = (float)(a - (deltaT * 0.5f));
+        float velMul = (float)((double)(velA * velA) * dot_delta * fscal);
+        launch_ic_noise<float2>(c->A, np, seed, c->st);
+        if ((rc = fft3d_f32(c, c->fftic, c->A, c->twf_ic, -1))) return rc;
+        launch_ic_scale<float2>(c->A, np, d_ipk, ipk_bins, ipk_k_delta, c->p.rl, c->st);
+        for (int d = 0; d < 3; d++) {
+            launch_ic_displacement<float2>(c->A, c->B, np, d, delta, c->p.rl, c->st);
+            if ((rc = fft3d_f32(c, c->fftic, c->B, c->twf_ic, +1))) return rc;
+            launch_ic_place_split(c->B, c->pos, c->vel, np, c->ng, d, delta, velMul, c->st);
+        }
+    } else {
+        double velA = a - deltaT * 0.5;
+        double velMul = velA * velA * dot_delta * fscal;
+        double2* amp = (double2*)c->grad64;
+        launch_ic_noise<double2>(amp, np, seed, c->st);
+        if ((rc = fft3d_f64(c, c->fftic, amp, c->tw64_ic, -1))) return rc;
+        launch_ic_scale<double2>(amp, np, d_ipk, ipk_bins, ipk_k_delta, c->p.rl, c->st);
+        for (int d = 0; d < 3; d++) {
+            launch_ic_displacement<double2>(amp, c->grid64, np, d, delta, c->p.rl, c->st);
+            if ((rc = fft3d_f64(c, c->fftic, c->grid64, c->tw64_ic, +1))) return rc;
+            launch_ic_place_f64(c->grid64, c->pos64, c->vel64, np, c->ng, d, delta, velMul, c->st);
+        }
+    }
+    LAUNCHCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    (void)hipFree(d_ipk);
+    c->have_particles = true;
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+// ------------------------------------------------------------ step parts ----
+int hgpm_update_positions(hgpm_ctx* c, double prefactor) {
+    if (!c) return HGPM_ERR_ARG;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (c->prec == HGPM_FP32) {
+        Timed t(c, "drift");
+        launch_drift(c->pos, c->vel, c->N, c->ng, (float)prefactor, c->st);
+    } else {
+        Timed t(c, "f64_drift");
+        launch_f64_drift(c->pos64, c->vel64, c->N, c->ng, (double)(float)prefactor, c->st);
+    }
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+int hgpm_cic(hgpm_ctx* c) {
+    if (!c) return HGPM_ERR_ARG;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    rc = c->prec == HGPM_FP32 ? f32_deposit(c) : f64_deposit(c, c->pos64);
+    if (rc) return rc;
+    c->have_density = true;
+    c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_get_density(hgpm_ctx* c, int64_t* rho) {
+    if (!c || !rho) return fail(c, HGPM_ERR_ARG, "hgpm_get_density: null argument");
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "int64 density needs an FP32 context");
+    if (!c->have_density) return fail(c, HGPM_ERR_STATE, "no density: call hgpm_cic first");
+    HIPCHK(c, hipMemcpyAsync(rho, c->B, sizeof(int64_t) * c->G, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return HGPM_OK;
+}
+
+int hgpm_get_density_f64(hgpm_ctx* c, double* rho) {
+    if (!c || !rho) return fail(c, HGPM_ERR_ARG, "hgpm_get_density_f64: null argument");
+    if (!c->have_density) return fail(c, HGPM_ERR_STATE, "no density: call hgpm_cic first");
+    if (c->prec == HGPM_FP32) {
+        std::vector<int64_t> q(c->G);
+        HIPCHK(c, hipMemcpyAsync(q.data(), c->B, sizeof(int64_t) * c->G, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int64_t i = 0; i < c->G; i++) rho[i] = (double)q[i] * (1.0 / 16777216.0);
+    } else {
+        double* tmp = (double*)c->grad64; // scratch
+        launch_f64_density_to_real(c->grid64, tmp, c->G, c->st);
+        LAUNCHCHK(c);
+        HIPCHK(c, hipMemcpyAsync(rho, tmp, sizeof(double) * c->G, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    return HGPM_OK;
+}
+
+int hgpm_set_density(hgpm_ctx* c, const int64_t* rho) {
+    if (!c || !rho) return fail(c, HGPM_ERR_ARG, "hgpm_set_density: null argument");
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "int64 density needs an FP32 context");
+    long long tot = 0;
+    for (int64_t i = 0; i < c->G; i++) tot += rho[i];
+    unsigned long long ut = (unsigned long long)tot;
+    HIPCHK(c, hipMemcpyAsync(c->B, rho, sizeof(int64_t) * c->G, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->total, &ut, sizeof(ut), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_density = true;
+    c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_set_density_f64(hgpm_ctx* c, const double* rho) {
+    if (!c || !rho) return fail(c, HGPM_ERR_ARG, "hgpm_set_density_f64: null argument");
+    if (c->prec != HGPM_FP64) return fail(c, HGPM_ERR_UNSUPPORTED, "FP64 density needs an FP64 context");
+    std::vector<double> g(2 * (size_t)c->G, 0.0);
+    for (int64_t i = 0; i < c->G; i++) g[2 * i] = rho[i];
+    HIPCHK(c, hipMemcpyAsync(c->grid64, g.data(), sizeof(double) * 2 * c->G, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_density = true;
+    c->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_solve_gradient(hgpm_ctx* c) {
+    if (!c) return HGPM_ERR_ARG;
+    if (!c->have_density) return fail(c, HGPM_ERR_STATE, "no density: call hgpm_cic first");
+    int rc = c->prec == HGPM_FP32 ? f32_solve(c) : f64_solve(c);
+    if (rc) return rc;
+    c->have_density = false;
+    c->have_force = true;
+    return HGPM_OK;
+}
+
+int hgpm_get_force(hgpm_ctx* c, float* f) {
+    if (!c || !f) return fail(c, HGPM_ERR_ARG, "hgpm_get_force: null argument");
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "use hgpm_get_force_f64 on an FP64 context");
+    if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
+    float* tmp = nullptr;
+    int rc = dalloc(c, &tmp, sizeof(float) * 3 * c->G, "staging");
+    if (rc) return rc;
+    c->bytes -= sizeof(float) * 3 * c->G;
+    launch_force_aos(c->B, c->fz, tmp, c->G, c->st);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(f, tmp, sizeof(float) * 3 * c->G, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    (void)hipFree(tmp);
+    return HGPM_OK;
+}
+
+int hgpm_get_force_f64(hgpm_ctx* c, double* f) {
+    if (!c || !f) return fail(c, HGPM_ERR_ARG, "hgpm_get_force_f64: null argument");
+    if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
+    if (c->prec == HGPM_FP32) {
+        std::vector<float> t(3 * (size_t)c->G);
+        int rc = hgpm_get_force(c, t.data());
+        if (rc) return rc;
+        for (size_t i = 0; i < t.size(); i++) f[i] = t[i];
+        return HGPM_OK;
+    }
+    double* tmp = nullptr;
+    int rc = dalloc(c, &tmp, sizeof(double) * 3 * c->G, "staging");
+    if (rc) return rc;
+    c->bytes -= sizeof(double) * 3 * c->G;
+    launch_f64_grad_to_force(c->grad64, tmp, c->G, c->st);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(f, tmp, sizeof(double) * 3 * c->G, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    (void)hipFree(tmp);
+    return HGPM_OK;
+}
+
+int hgpm_update_velocities(hgpm_ctx* c, double deltaT, double fscal) {
+    if (!c) return HGPM_ERR_ARG;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
+    if (c->prec == HGPM_FP32) {
+        Timed t(c, "gather_kick");
+        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->st);
+    } else {
+        Timed t(c, "f64_icic");
+        launch_f64_icic(c->pos64, c->vel64, c->grad64, c->N, c->ng, deltaT, fscal, c->st);
+    }
+    LAUNCHCHK(c);
+    return HGPM_OK;
+}
+
+int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) {
+    if (!c) return HGPM_ERR_ARG;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
+    if (c->prec == HGPM_FP32) {
+        Timed t(c, "gather_kick_drift");
+        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor, c->st);
+        LAUNCHCHK(c);
+        return HGPM_OK;
+    }
+    rc = hgpm_update_velocities(c, deltaT, fscal);
+    if (rc) return rc;
+    return hgpm_update_positions(c, prefactor);
+}
+
+int hgpm_step(hgpm_ctx* c, const hgpm_step_coeffs* k) {
+    if (!c || !k) return fail(c, HGPM_ERR_ARG, "hgpm_step: null argument");
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (!k->skip_drift0 && (rc = hgpm_update_positions(c, k->drift0))) return rc;
+    if ((rc = hgpm_cic(c))) return rc;
+    if ((rc = hgpm_solve_gradient(c))) return rc;
+    return hgpm_kick_drift(c, k->kick_dt, k->kick_fscal, k->drift1);
+}
+
+// ------------------------------------------------------------------- FFT ----
+int hgpm_fft_device(hgpm_ctx* c, void* dev, int direction) {
+    if (!c || !dev || (direction != -1 && direction != 1)) return fail(c, HGPM_ERR_ARG, "hgpm_fft_device: bad arguments");
+    return c->prec == HGPM_FP32 ? fft3d_f32(c, c->fft, (float2*)dev, c->twf, direction)
+                                : fft3d_f64(c, c->fft, (double2*)dev, c->tw64, direction);
+}
+
+int hgpm_fft_host(hgpm_ctx* c, void* grid, int direction) {
+    if (!c || !grid || (direction != -1 && direction != 1)) return fail(c, HGPM_ERR_ARG, "hgpm_fft_host: bad arguments");
+    size_t bytes = (c->prec == HGPM_FP32 ? sizeof(float2) : sizeof(double2)) * (size_t)c->G;
+    void* d = c->prec == HGPM_FP32 ? (void*)c->A : (void*)c->grid64;
+    HIPCHK(c, hipMemcpyAsync(d, grid, bytes, hipMemcpyHostToDevice, c->st));
+    int rc = hgpm_fft_device(c, d, direction);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(grid, d, bytes, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+// ------------------------------------------------------------------- P(k) ----
+int hgpm_power_spectrum_grid(hgpm_ctx* c, int fold, double* k, double* pk, int64_t* count) {
+    if (!c) return HGPM_ERR_ARG;
+    if (fold < 0 || fold > c->p.pk_folds) return fail(c, HGPM_ERR_ARG, "fold out of range");
+    if (!c->have_density) return fail(c, HGPM_ERR_STATE, "no density: call hgpm_cic or hgpm_set_density first");
+    int rc;
+    if (c->prec == HGPM_FP32) {
+        {
+            Timed t(c, "pk_fwd_z_density");
+            c->fft->fwd_z_density((const long long*)c->B, c->total, c->A, c->twf, c->st);
+        }
+        {
+            Timed t(c, "pk_fwd_yx");
+            c->fft->pass_f32(c->A, 1, -1, c->twf, c->st);
+            c->fft->pass_f32(c->A, 0, -1, c->twf, c->st);
+        }
+        LAUNCHCHK(c);
+    } else {
+        if ((rc = fft3d_f64(c, c->fft, c->grid64, c->tw64, -1))) return rc;
+    }
+    c->have_density = false;
+    c->have_force = false;
+    return pk_bin_spectrum(c, fold, k, pk, count);
+}
+
+int hgpm_power_spectrum(hgpm_ctx* c, int fold, double* k, double* pk, int64_t* count) {
+    if (!c) return HGPM_ERR_ARG;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (fold < 0 || fold > c->p.pk_folds) return fail(c, HGPM_ERR_ARG, "fold out of range");
+    if (c->prec == HGPM_FP32) {
+        if (fold == 0) {
+            if ((rc = f32_deposit(c))) return rc;
+        } else {
+            // fold a copy of the positions (x -> 2^fold x mod ng, exact in the split
+            // layout) held in a temporary buffer
+            PosSplitH* tmp = nullptr;
+            if ((rc = dalloc(c, &tmp, sizeof(PosSplitH) * c->N, "fold copy"))) return rc;
+            c->bytes -= sizeof(PosSplitH) * c->N;
+            HIPCHK(c, hipMemcpyAsync(tmp, c->pos, sizeof(PosSplitH) * c->N, hipMemcpyDeviceToDevice, c->st));
+            for (int f = 0; f < fold; f++) launch_fold_split(tmp, c->N, c->ng, c->st);
+            PosSplitH* keep = c->pos;
+            c->pos = tmp;
+            rc = f32_deposit(c);
+            c->pos = keep;
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            (void)hipFree(tmp);
+            if (rc) return rc;
+        }
+    } else {
+        double* src = c->pos64;
+        double* tmp = nullptr;
+        if (fold > 0) {
+            if ((rc = dalloc(c, &tmp, sizeof(double) * 3 * c->N, "fold copy"))) return rc;
+            c->bytes -= sizeof(double) * 3 * c->N;
+            HIPCHK(c, hipMemcpyAsync(tmp, c->pos64, sizeof(double) * 3 * c->N, hipMemcpyDeviceToDevice, c->st));
+            for (int f = 0; f < fold; f++) launch_f64_fold(tmp, c->N, c->ng, c->st);
+            src = tmp;
+        }
+        rc = f64_deposit(c, src);
+        if (tmp) {
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            (void)hipFree(tmp);
+        }
+        if (rc) return rc;
+    }
+    c->have_density = true;
+    return hgpm_power_spectrum_grid(c, fold, k, pk, count);
+}
+
+// writePkOutput (src/power_spectrum/cuda/power.cu:13-28): "%g,%g,%d" lines of
+// non-empty, non-zero, non-NaN bins to <fname>.fold.<f>.
+int hgpm_write_power_spectrum(hgpm_ctx* c, const char* fname) {
+    if (!c || !fname) return fail(c, HGPM_ERR_ARG, "hgpm_write_power_spectrum: null argument");
+    const int nb = c->p.pk_bins;
+    std::vector<double> k(nb), pk(nb);
+    std::vector<int64_t> cnt(nb);
+    for (int f = 0; f <= c->p.pk_folds; f++) {
+        int rc = hgpm_power_spectrum(c, f, k.data(), pk.data(), cnt.data());
+        if (rc) return rc;
+        std::string fn = std::string(fname) + ".fold." + std::to_string(f);
+        FILE* fp = fopen(fn.c_str(), "w+");
+        if (!fp) return fail(c, HGPM_ERR_IO, "cannot write " + fn);
+        for (int i = 0; i < nb; i++) {
+            double out = pk[i];
+            if (out == out && cnt[i] != 0 && out != 0) fprintf(fp, "%g,%g,%d\n", k[i], out, (int)cnt[i]);
+        }
+        fclose(fp);
+    }
+    return HGPM_OK;
+}
+
+// writeOutput (src/simulation/cuda/io.cu:13-44): raw float3 positions then velocities.
+int hgpm_write_output(hgpm_ctx* c, const char* fname) {
+    if (!c || !fname) return fail(c, HGPM_ERR_ARG, "hgpm_write_output: null argument");
+    std::vector<float> p(3 * (size_t)c->N), v(3 * (size_t)c->N);
+    int rc = hgpm_get_particles(c, p.data(), v.data());
+    if (rc) return rc;
+    FILE* fp = fopen(fname, "w+");
+    if (!fp) return fail(c, HGPM_ERR_IO, std::string("cannot write ") + fname);
+    size_t a = fwrite(p.data(), sizeof(float), p.size(), fp);
+    size_t b = fwrite(v.data(), sizeof(float), v.size(), fp);
+    fclose(fp);
+    if (a != p.size() || b != v.size()) return fail(c, HGPM_ERR_IO, std::string("short write: ") + fname);
+    return HGPM_OK;
+}
+
+// ------------------------------------------------------------ timing ----
+int hgpm_set_timing(hgpm_ctx* c, int enabled) {
+    if (!c) return HGPM_ERR_ARG;
+    fold_pending(c);
+    c->timing = enabled != 0;
+    return HGPM_OK;
+}
+int hgpm_reset_timings(hgpm_ctx* c) {
+    if (!c) return HGPM_ERR_ARG;
+    fold_pending(c);
+    for (auto& s : c->stats) s = KStat{};
+    return HGPM_OK;
+}
+int hgpm_get_timings(hgpm_ctx* c, char* names, int names_len, double* total_ms, int64_t* launches, int max, int* n) {
+    if (!c || !n) return HGPM_ERR_ARG;
+    fold_pending(c);
+    int cnt = 0, off = 0;
+    for (size_t i = 0; i < c->names.size() && cnt < max; i++) {
+        const std::string& s = c->names[i];
+        if (names) {
+            if (off + (int)s.size() + 1 > names_len) break;
+            memcpy(names + off, s.c_str(), s.size() + 1);
+            off += (int)s.size() + 1;
+        }
+        if (total_ms) total_ms[cnt] = c->stats[i].ms;
+        if (launches) launches[cnt] = c->stats[i].launches;
+        cnt++;
+    }
+    *n = cnt;
+    return HGPM_OK;
+}
+int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
+    if (!c || !key) return HGPM_ERR_ARG;
+    return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key + " (" + std::to_string(value) + ")");
+}
+int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
+    if (!c || !key || !value) return HGPM_ERR_ARG;
+    return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
+}
+
+} // extern "C"

@@ -1,0 +1,49 @@
+// common.h — shared device/host helpers of the HIP engine (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef HGPM_PI
+#define HGPM_PI 3.14159265358979323846
+#endif
+
+// Complex element of the FP32 / FP64 grids (interleaved re, im).
+template <typename T> struct Cx;
+template <> struct Cx<float> { typedef float2 type; };
+template <> struct Cx<double> { typedef double2 type; };
+
+template <typename V> __host__ __device__ __forceinline__ V cmake(decltype(V().x) r, decltype(V().x) i) {
+    V v; v.x = r; v.y = i; return v;
+}
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return cmake<V>(a.x + b.x, a.y + b.y); }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return cmake<V>(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+    return cmake<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by -i (DIR<0) or +i (DIR>0)
+template <int DIR, typename V> __device__ __forceinline__ V cmul_mi(V a) {
+    return DIR < 0 ? cmake<V>(a.y, -a.x) : cmake<V>(-a.y, a.x);
+}
+
+// Map a hardware workgroup id to a logical one so that each XCD (8 per MI355X,
+// workgroups dispatched round-robin) receives one contiguous range of logical ids:
+// neighbouring particles / tiles then share that XCD's L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
+    const int64_t NX = 8;
+    int64_t x = bid % NX, i = bid / NX;
+    int64_t per = nb / NX, rem = nb % NX;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+// Grid index of cell (x, y, z): serial::get_index order (src/haccgpm.hpp:423-429).
+__device__ __forceinline__ int64_t gidx(int ng, int x, int y, int z) {
+    return ((int64_t)x * ng + y) * ng + z;
+}
+
+// Particle position in the build layout: int32 cell + FP32 offset in [0,1).
+struct __align__(8) PosSplit {
+    int cx, cy, cz;
+    float fx, fy, fz;
+};
+
+#define HGPM_WAVE 64

@@ -1,0 +1,86 @@
+// internal.h — declarations shared by the engine's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace hgpm {
+
+// Transform-length-specific operations, one table per supported length.
+struct FftOps {
+    int n;
+    int stages;
+    void (*twiddles)(double* fwd, double* rev); // stages*n complex each
+    void (*pass_f32)(float2* d, int axis, int dir, const float2* tw, hipStream_t s);
+    void (*pass_f64)(double2* d, int axis, int dir, const double2* tw, hipStream_t s);
+    void (*fwd_z_density)(const long long* rho, const unsigned long long* total, float2* out,
+                          const float2* tw, hipStream_t s);
+    void (*inv_z_real)(const float2* in, float* out, const float2* tw, hipStream_t s);
+    void (*x_solve)(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk,
+                    const float* hs, hipStream_t s);
+};
+
+#define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+#define HGPM_DECL_OPS(n) extern const FftOps fft_ops_##n;
+HGPM_FFT_SIZES(HGPM_DECL_OPS)
+#undef HGPM_DECL_OPS
+
+inline const FftOps* fft_ops(int n) {
+    switch (n) {
+#define HGPM_CASE(n) \
+    case n: return &fft_ops_##n;
+        HGPM_FFT_SIZES(HGPM_CASE)
+#undef HGPM_CASE
+    default: return nullptr;
+    }
+}
+
+// ---- FP32 particle kernels (k_particles.hip) ----
+struct PosSplitH { int cx, cy, cz; float fx, fy, fz; };
+void launch_zero_i64(long long* p, size_t n, hipStream_t s);
+void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+void launch_drift(PosSplitH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
+void launch_deposit(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
+                    unsigned long long* total, hipStream_t s);
+void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
+                        double deltaT, double fscal, int do_drift, float pf, hipStream_t s);
+void launch_split_from_float3(const float* pos3, PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
+void launch_split_to_float3(const PosSplitH* pos, float* pos3, int64_t np3, hipStream_t s);
+void launch_fold_split(PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
+void launch_force_aos(const float2* fxy, const float* fz, float* out, int64_t g3, hipStream_t s);
+void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s);
+
+// ---- P(k) (k_power.hip) ----
+// bins of a complex spectrum (ng^3) with FP64 deterministic sums.
+template <typename V>
+void launch_power_bins(const V* spec, int ng, double rl_fold, double minK, double binDelta, int nbins,
+                       const double* win, double scale, double* partial_sums, long long* counts,
+                       int nblocks, double* sums_out, hipStream_t s);
+int power_partial_blocks(int ng);
+
+// ---- IC (k_ic.hip) ----
+template <typename V>
+void launch_ic_noise(V* grid, int np, uint32_t seed, hipStream_t s);
+template <typename V>
+void launch_ic_scale(V* grid, int np, const double* ipk, int ipk_bins, double k_delta, double rl, hipStream_t s);
+template <typename V>
+void launch_ic_displacement(const V* amp, V* out, int np, int dim, double delta, double rl, hipStream_t s);
+void launch_ic_place_split(const float2* s_grid, PosSplitH* pos, float* vel, int np, int ng, int dim,
+                           double delta, float vel_mul, hipStream_t s);
+void launch_ic_place_f64(const double2* s_grid, double* pos, double* vel, int np, int ng, int dim,
+                         double delta, double vel_mul, hipStream_t s);
+
+// ---- FP64 plain path (k_fp64.hip) ----
+void launch_f64_drift(double* pos, const double* vel, int64_t np3, int ng, double pf, hipStream_t s);
+void launch_f64_zero(double* p, size_t n, hipStream_t s);
+void launch_f64_cic(const double* pos, int64_t np3, int ng, double mass, double2* grid, hipStream_t s);
+void launch_f64_copy_to_grad(const double2* grid, double* grad4, int64_t g3, hipStream_t s);
+void launch_f64_kspace(double2* grid, const double* grad4, int ng, int dim, hipStream_t s);
+void launch_f64_real_to_grad(const double2* grid, double* grad4, int64_t g3, int dim, hipStream_t s);
+void launch_f64_icic(const double* pos, double* vel, const double* grad4, int64_t np3, int ng, double deltaT,
+                     double fscal, hipStream_t s);
+void launch_f64_fold(double* pos, int64_t np3, int ng, hipStream_t s);
+void launch_f64_density_to_real(const double2* grid, double* out, int64_t g3, hipStream_t s);
+void launch_f64_grad_to_force(const double* grad4, double* out, int64_t g3, hipStream_t s);
+
+} // namespace hgpm

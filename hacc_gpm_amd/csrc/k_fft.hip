@@ -1,0 +1,175 @@
+// k_fft.hip — 3-D FFT passes for one transform length N (compiled once per N with
+// -DHGPM_FFT_N=<N>).  Replaces the cuFFT C2C/Z2Z calls of src/fft_tools/cuda/ffts.cu
+// (plans :60-130, forward :244-294, inverse :397-445: unnormalised, in place,
+// x-major ng^3 grids) and fuses the spectral solve of src/pm_solver/solver.cpp:145-170
+// into the transform passes.
+//
+// Passes (grid index (x*N + y)*N + z, z contiguous):
+//   rows  : 1-D transforms along z; a workgroup loads RC whole rows (coalesced).
+//   cols  : 1-D transforms along y (stride N) or x (stride N^2); a workgroup loads CC
+//           adjacent z values of N rows (CC*8 contiguous bytes per row).
+//   x_solve (FP32): forward transform along x, Green's function x central-difference
+//           gradient for all three axes, and the inverse transforms along x of
+//           P = Fx^ + i Fy^ and Q = Fz^, without the spectrum leaving registers.
+#include "fft.h"
+#include "internal.h"
+
+#ifndef HGPM_FFT_N
+#error "compile with -DHGPM_FFT_N=<transform length>"
+#endif
+
+namespace {
+constexpr int N = HGPM_FFT_N;
+constexpr int NT = 256;
+
+// rows per row tile / columns per column tile (FP32, FP64)
+constexpr int rc32() { return N >= 768 ? 4 : (N >= 384 ? 8 : (N >= 192 ? 16 : (N >= 96 ? 32 : 64))); }
+constexpr int cc32() { return N >= 768 ? 8 : (N >= 192 ? 16 : 32); }
+constexpr int rc64() { return rc32() > 1 ? rc32() / 2 : 1; }
+constexpr int cc64() { return N >= 768 ? 4 : (N >= 192 ? 8 : 16); }
+
+// ------------------------------------------------------------- rows ----
+template <typename V, int RC, int DIR>
+__global__ __launch_bounds__(NT) void k_rows_c2c(V* __restrict d, const V* __restrict tw) {
+    __shared__ V lds[lds_elems<N, RC>()];
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    fft_tile<V, N, RC, NT, DIR, false, true>(
+        [&](int, int, int n, int c) { return d[(row0 + c) * N + n]; },
+        [&](int, int, int n, int c, V v) { d[(row0 + c) * N + n] = v; }, lds, tw);
+}
+
+// Forward transform along z of the int64 fixed-point density, mean removed in FP64
+// (the k = 0 normalisation) before the FP32 rounding.
+template <int RC>
+__global__ __launch_bounds__(NT) void k_rows_density(const long long* __restrict rho,
+                                                     const unsigned long long* __restrict total,
+                                                     float2* __restrict out, const float2* __restrict tw) {
+    __shared__ float2 lds[lds_elems<N, RC>()];
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    const double mean = (double)(long long)(*total) / ((double)N * N * N);
+    fft_tile<float2, N, RC, NT, -1, false, true>(
+        [&](int, int, int n, int c) {
+            double q = (double)rho[(row0 + c) * N + n];
+            return make_float2((float)((q - mean) * (1.0 / 16777216.0)), 0.0f);
+        },
+        [&](int, int, int n, int c, float2 v) { out[(row0 + c) * N + n] = v; }, lds, tw);
+}
+
+// Inverse transform along z keeping the real part (Fz).
+template <int RC>
+__global__ __launch_bounds__(NT) void k_rows_real_out(const float2* __restrict in, float* __restrict out,
+                                                      const float2* __restrict tw) {
+    __shared__ float2 lds[lds_elems<N, RC>()];
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    fft_tile<float2, N, RC, NT, +1, false, true>(
+        [&](int, int, int n, int c) { return in[(row0 + c) * N + n]; },
+        [&](int, int, int n, int c, float2 v) { out[(row0 + c) * N + n] = v.x; }, lds, tw);
+}
+
+// ------------------------------------------------------------- cols ----
+// axis 1 (y): tile = (x, z-chunk), element (n=y, c) at x N^2 + n N + z0 + c
+// axis 0 (x): tile = (y, z-chunk), element (n=x, c) at n N^2 + y N + z0 + c
+template <typename V, int CC, int DIR>
+__global__ __launch_bounds__(NT) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis) {
+    __shared__ V lds[lds_elems<N, CC>()];
+    const int tile = blockIdx.x;
+    const int outer = tile / (N / CC), zc = tile % (N / CC);
+    const size_t NN = (size_t)N * N;
+    const size_t base = axis == 1 ? (size_t)outer * NN + (size_t)zc * CC : (size_t)outer * N + (size_t)zc * CC;
+    const size_t stride = axis == 1 ? (size_t)N : NN;
+    fft_tile<V, N, CC, NT, DIR, false, false>(
+        [&](int, int, int n, int c) { return d[base + n * stride + c]; },
+        [&](int, int, int n, int c, V v) { d[base + n * stride + c] = v; }, lds, tw);
+}
+
+// ---------------------------------------------------------- x_solve ----
+// In: a = rho^ after the z and y forward passes.  Out: p = IFFT_x(Fx^ + i Fy^),
+// a = IFFT_x(Fz^).  F_d^ = -i sin(k_d) G rho^ with
+// G = (0.5/N^3)/(sum_d cos k_d - 3) = -(0.25/N^3)/sum_d sin^2(k_d/2)
+// (kspace_solve_gradient, src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens,
+// src/pm_solver/cuda/calcGreens.hpp:11-26), G(0) = 0.  sk[m] = sin(2 pi m_s/N) with
+// the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274); hs[m] = sin^2(pi m/N).
+template <int CC>
+__global__ __launch_bounds__(NT) void k_x_solve(float2* __restrict a, float2* __restrict p,
+                                                const float2* __restrict twf, const float2* __restrict twr,
+                                                const float* __restrict sk, const float* __restrict hs,
+                                                float gcoef) {
+    __shared__ float2 lds[lds_elems<N, CC>()];
+    const int tile = blockIdx.x;
+    const int y = tile / (N / CC), zc = tile % (N / CC);
+    const size_t NN = (size_t)N * N;
+    const size_t base = (size_t)y * N + (size_t)zc * CC;
+    typedef LastShape<N, CC, NT> LS;
+    float2 P[LS::NB][LS::R], Q[LS::NB][LS::R];
+    const float hy = hs[y], sy = sk[y];
+    fft_tile<float2, N, CC, NT, -1, false, false>(
+        [&](int, int, int n, int c) { return a[base + n * NN + c]; },
+        [&](int b, int r, int n, int c, float2 v) {
+            const int z = zc * CC + c;
+            const float ssum = (hs[n] + hy) + hs[z];
+            const float g = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
+            const float sx = sk[n], sz = sk[z];
+            const float gx = g * v.x, gy = g * v.y;
+            P[b][r] = make_float2(-sy * gx - sx * gy, -sy * gy + sx * gx);
+            Q[b][r] = make_float2(-sz * gy, sz * gx);
+        },
+        lds, twf);
+    fft_tile<float2, N, CC, NT, +1, true, false>(
+        [&](int b, int r, int, int) { return P[b][r]; },
+        [&](int, int, int n, int c, float2 v) { p[base + n * NN + c] = v; }, lds, twr);
+    fft_tile<float2, N, CC, NT, +1, true, false>(
+        [&](int b, int r, int, int) { return Q[b][r]; },
+        [&](int, int, int n, int c, float2 v) { a[base + n * NN + c] = v; }, lds, twr);
+}
+
+// --------------------------------------------------------- launchers ----
+template <typename V, int RC, int CC>
+void pass_impl(V* d, int axis, int dir, const V* tw, hipStream_t s) {
+    if (axis == 2) {
+        unsigned nb = (unsigned)((size_t)N * N / RC);
+        if (dir < 0) k_rows_c2c<V, RC, -1><<<nb, NT, 0, s>>>(d, tw);
+        else k_rows_c2c<V, RC, +1><<<nb, NT, 0, s>>>(d, tw);
+    } else {
+        unsigned nb = (unsigned)((size_t)N * (N / CC));
+        if (dir < 0) k_cols_c2c<V, CC, -1><<<nb, NT, 0, s>>>(d, tw, axis);
+        else k_cols_c2c<V, CC, +1><<<nb, NT, 0, s>>>(d, tw, axis);
+    }
+}
+
+void pass_f32(float2* d, int axis, int dir, const float2* tw, hipStream_t s) {
+    pass_impl<float2, rc32(), cc32()>(d, axis, dir, tw, s);
+}
+void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
+    pass_impl<double2, rc64(), cc64()>(d, axis, dir, tw, s);
+}
+void fwd_z_density(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
+                   hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * N / rc32());
+    k_rows_density<rc32()><<<nb, NT, 0, s>>>(rho, total, out, tw);
+}
+void inv_z_real(const float2* in, float* out, const float2* tw, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * N / rc32());
+    k_rows_real_out<rc32()><<<nb, NT, 0, s>>>(in, out, tw);
+}
+void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
+             hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * (N / cc32()));
+    const float gcoef = (float)(0.25 / ((double)N * N * N));
+    k_x_solve<cc32()><<<nb, NT, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
+}
+void twiddles(double* fwd, double* rev) {
+    make_twiddles<N, false>(fwd);
+    make_twiddles<N, true>(rev);
+}
+} // namespace
+
+// The operation table holds host function pointers: host compilation pass only.
+#ifndef __HIP_DEVICE_COMPILE__
+namespace hgpm {
+#define HGPM_CAT2(a, b) a##b
+#define HGPM_CAT(a, b) HGPM_CAT2(a, b)
+extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
+    N, Plan<N>::S, twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
+};
+} // namespace hgpm
+#endif

@@ -1,0 +1,44 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (parity tests through the C ABI)")
+    config.addinivalue_line("markers", "slow: longer CPU/GPU runs")
+
+
+@pytest.fixture(scope="session")
+def hg():
+    import hacc_gpm_amd
+    hacc_gpm_amd.lib()
+    return hacc_gpm_amd
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    oracle.L()
+    return oracle
+
+
+def rand_split(rng, n, ng, clustered=False):
+    """Random particles in the split layout (cell int32, offset float32 in [0,1))."""
+    if clustered:
+        centre = rng.integers(0, ng, 3)
+        x = (centre + rng.normal(0, 1.5, (n, 3))) % ng
+    else:
+        x = rng.random((n, 3)) * ng
+    cell = np.floor(x).astype(np.int32) % ng
+    off = (x - np.floor(x)).astype(np.float32)
+    bad = off >= 1.0
+    off[bad] = 0.0
+    cell[bad] = (cell[bad] + 1) % ng
+    return cell, off

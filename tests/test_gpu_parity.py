@@ -1,0 +1,335 @@
+"""GPU parity tests: the HIP engine (through the C ABI of libhgpm.so) against the CPU
+oracle on the same inputs.
+
+Bars:
+  - int64 density, cell indices, offsets and velocities given the same force grid:
+    bit-exact (integer / FP32 recipes without FMA contraction on both sides);
+  - FFT: relative L2 error vs FP64 <= 2e-6 (FP32) / 1e-13 (FP64);
+  - forces (FP32 solve) vs FP64 oracle: relative RMS <= 2e-5;
+  - P(k): counts exact, values relative <= 1e-4 (FP32 spectrum);
+  - full runs: stated in each test.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rand_split
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def mk(hg, ng, np_=None, rl=None, prec=0, **kw):
+    np_ = np_ or ng
+    rl = rl or 2.0 * ng
+    return hg.MemoryManager(hg.make_params(ng, np_, rl, 50.0, 0.0, 100, precision=prec, **kw))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+# --------------------------------------------------------------- deposit ----
+@pytest.mark.parametrize("ng,clustered", [(32, False), (64, False), (64, True), (96, True)])
+def test_deposit_bit_exact(hg, orc, ng, clustered):
+    case(f"deposit ng={ng} clustered={clustered}")
+    rng = np.random.default_rng(ng + clustered)
+    with mk(hg, ng) as m:
+        cell, off = rand_split(rng, m.N, ng, clustered)
+        # particles on the periodic seam and exactly on lattice points
+        cell[:5] = ng - 1
+        off[:5] = np.float32(1.0) - np.float32(2.0 ** -24)
+        off[5:9] = 0.0
+        vel = np.zeros((m.N, 3), np.float32)
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        got = m.get_density()
+        ref = orc.cic_split(ng, cell, off)
+        assert np.array_equal(got, ref)
+        m.cic()  # deterministic across runs
+        assert np.array_equal(m.get_density(), got)
+
+
+def test_deposit_mass_ratio(hg, orc):
+    case("deposit np=32 ng=64 mass 8")
+    rng = np.random.default_rng(1)
+    with mk(hg, 64, 32) as m:
+        cell, off = rand_split(rng, m.N, 64)
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        assert np.array_equal(m.get_density(), orc.cic_split(64, cell, off, 8.0))
+
+
+# ------------------------------------------------------------------- FFT ----
+@pytest.mark.parametrize("ng", [32, 64, 96, 128, 192])
+def test_fft_fp32(hg, ng):
+    case(f"fft fp32 ng={ng}")
+    rng = np.random.default_rng(ng)
+    x = (rng.standard_normal((ng,) * 3) + 1j * rng.standard_normal((ng,) * 3)).astype(np.complex64)
+    with mk(hg, ng) as m:
+        X = m.fft(x, -1)
+        ref = np.fft.fftn(x.astype(np.complex128))
+        assert rel_l2(X, ref) < 2e-6
+        back = m.fft(X, +1)
+        assert rel_l2(back, ng ** 3 * x.astype(np.complex128)) < 4e-6
+
+
+@pytest.mark.parametrize("ng", [32, 64, 96])
+def test_fft_fp64(hg, ng):
+    case(f"fft fp64 ng={ng}")
+    rng = np.random.default_rng(ng + 7)
+    x = rng.standard_normal((ng,) * 3) + 1j * rng.standard_normal((ng,) * 3)
+    with mk(hg, ng, prec=1) as m:
+        X = m.fft(x, -1)
+        assert rel_l2(X, np.fft.fftn(x)) < 1e-13
+        assert rel_l2(m.fft(X, +1), ng ** 3 * x) < 1e-13
+
+
+# ----------------------------------------------------------------- solve ----
+@pytest.mark.parametrize("ng", [32, 64, 96])
+def test_solve_fp32_vs_fp64_oracle(hg, orc, ng):
+    case(f"solve fp32 ng={ng}")
+    rng = np.random.default_rng(ng + 3)
+    with mk(hg, ng) as m:
+        cell, off = rand_split(rng, m.N, ng, clustered=False)
+        # add structure: pull particles toward a few centres
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        q = m.get_density()
+        m.solve_gradient()
+        F = m.get_force().astype(np.float64)
+    Fo = orc.solve_gradient(ng, q * 2.0 ** -24)
+    err = np.sqrt(((F - Fo) ** 2).mean()) / np.sqrt((Fo ** 2).mean())
+    assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("ng", [32, 64])
+def test_solve_fp64_path(hg, orc, ng):
+    case(f"solve fp64 ng={ng}")
+    rng = np.random.default_rng(ng + 5)
+    rho = 1 + 0.3 * rng.standard_normal((ng,) * 3)
+    with mk(hg, ng, prec=1) as m:
+        m.set_density_f64(rho)
+        m.solve_gradient()
+        F = m.get_force()
+    Fo = orc.solve_gradient(ng, rho)
+    assert rel_l2(F, Fo) < 1e-12
+
+
+# --------------------------------------------------------- kick / drift ----
+@pytest.mark.parametrize("ng", [32, 64])
+def test_kick_drift_bit_exact(hg, orc, ng):
+    case(f"kick/drift ng={ng}")
+    rng = np.random.default_rng(ng + 11)
+    with mk(hg, ng) as m:
+        cell, off = rand_split(rng, m.N, ng)
+        vel = rng.normal(0, 3, (m.N, 3)).astype(np.float32)
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        m.solve_gradient()
+        F = m.get_force()
+        dT, fs, pf = 0.0123, 1.87, 0.75
+        # kick alone
+        m.update_velocities(dT, fs)
+        c1, o1, v1 = m.get_particles_split()
+        vref = orc.kick_split(ng, cell, off, vel, F, dT, fs)
+        assert np.array_equal(v1, vref)
+        assert np.array_equal(c1, cell) and np.array_equal(o1, off)
+        # fused kick + drift from the same start
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        m.solve_gradient()
+        m.kick_drift(dT, fs, pf)
+        c2, o2, v2 = m.get_particles_split()
+        cref, oref = orc.drift_split(ng, cell, off, vref, pf)
+        assert np.array_equal(v2, vref)
+        assert np.array_equal(c2, cref) and np.array_equal(o2, oref)
+        # standalone drift
+        m.update_positions(-0.4)
+        c3, o3, _ = m.get_particles_split()
+        c3r, o3r = orc.drift_split(ng, cref, oref, vref, -0.4)
+        assert np.array_equal(c3, c3r) and np.array_equal(o3, o3r)
+
+
+def test_fp64_kick_drift(hg, orc):
+    case("kick/drift fp64 ng=32")
+    ng = 32
+    rng = np.random.default_rng(2)
+    pos = rng.random((ng ** 3, 3)) * ng
+    vel = rng.normal(0, 2, (ng ** 3, 3))
+    with mk(hg, ng, prec=1) as m:
+        m.set_particles_f64(pos, vel)
+        m.cic()
+        rho = m.get_density_f64()
+        assert np.abs(rho - orc.cic_f64(ng, pos)).max() < 1e-10
+        m.solve_gradient()
+        F = m.get_force()
+        m.update_velocities(0.02, 1.3)
+        _, v = m.get_particles_f64()
+        np.testing.assert_allclose(v, orc.kick_f64(ng, pos, vel, F, 0.02, 1.3), rtol=0, atol=1e-12)
+        m.update_positions(0.5)
+        p2, _ = m.get_particles_f64()
+        pf = float(np.float32(0.5))
+        np.testing.assert_allclose(p2, orc.drift_f64(ng, pos, v, pf), rtol=0, atol=1e-11)
+
+
+# ------------------------------------------------------------------ P(k) ----
+@pytest.mark.parametrize("ng,folds", [(32, 0), (64, 0), (64, 2)])
+def test_power_spectrum(hg, orc, ng, folds):
+    case(f"power ng={ng} folds={folds}")
+    rng = np.random.default_rng(ng + folds)
+    nb = 24
+    with mk(hg, ng, pk_bins=nb, pk_folds=folds) as m:
+        cell, off = rand_split(rng, m.N, ng, clustered=True)
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        for f in range(folds + 1):
+            k, pk, cnt = m.power_spectrum(f)
+            c, o = cell, off
+            for _ in range(f):
+                c, o = orc.fold_split(ng, c, o)
+            q = orc.cic_split(ng, c, o)
+            ko, pko, cnto = orc.power_grid(ng, 2.0 * ng, folds, f, nb, q * 2.0 ** -24)
+            assert np.array_equal(cnt, cnto)
+            np.testing.assert_array_equal(k, ko)
+            nz = cnto > 0
+            assert np.abs(pk[nz] / pko[nz] - 1).max() < 1e-4
+        # deterministic FP64 sums
+        k2, pk2, cnt2 = m.power_spectrum(folds)
+        assert np.array_equal(pk2, pk) and np.array_equal(cnt2, cnt)
+
+
+def test_power_spectrum_fp64(hg, orc):
+    case("power fp64 ng=32")
+    ng, nb = 32, 16
+    rng = np.random.default_rng(8)
+    pos = rng.random((ng ** 3, 3)) * ng
+    with mk(hg, ng, prec=1, pk_bins=nb, pk_folds=1) as m:
+        m.set_particles_f64(pos, np.zeros_like(pos))
+        for f in (0, 1):
+            k, pk, cnt = m.power_spectrum(f)
+            pp = pos.copy()
+            for _ in range(f):
+                pp = orc.fold_f64(ng, pp)
+            ko, pko, cnto = orc.power_grid(ng, 64.0, 1, f, nb, orc.cic_f64(ng, pp))
+            assert np.array_equal(cnt, cnto)
+            nz = cnto > 0
+            assert np.abs(pk[nz] / pko[nz] - 1).max() < 1e-10
+
+
+# -------------------------------------------------------------------- IC ----
+def ipk_for(hg, p, tmp):
+    fn = os.path.join(tmp, "eh.ipk")
+    hg.write_ipk_eh(p, fn, 50.0, 50001)
+    return hg.read_ipk(fn)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_initial_conditions(hg, orc, tmp_path, prec):
+    case(f"ic ng=64 prec={prec}")
+    p = hg.make_params(64, 64, 128.0, 50.0, 0.0, 100, seed=21082001, precision=prec)
+    hdr, vals = ipk_for(hg, p, str(tmp_path))
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    with hg.MemoryManager(p) as m:
+        D, dD = hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
+        if prec == 0:
+            c, o, v = m.get_particles_split()
+            x = c + o.astype(np.float64)
+        else:
+            x, v = m.get_particles_f64()
+    xo, vo = orc.ic_f64(64, 64, 128.0, 21082001, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
+    d = (x - xo + 32) % 64 - 32
+    disp = (xo - np.indices((64,) * 3).reshape(3, -1).T + 32) % 64 - 32
+    # FP32: float noise grid, FP32 transforms and float k (reference rounding);
+    # FP64: the oracle's recipe in FP64
+    tol_x, tol_v = (1e-4, 1e-4) if prec == 0 else (1e-9, 1e-9)
+    print(f"[result] ic max|dx| {np.abs(d).max():.3e} max rel dv {np.abs(v - vo).max() / np.abs(vo).max():.3e}")
+    assert np.abs(d).max() < tol_x
+    assert np.abs(v - vo).max() < tol_v * np.abs(vo).max()
+    assert np.sqrt((disp ** 2).mean()) > 1e-2  # non-trivial displacements
+
+
+# -------------------------------------------------------------- full run ----
+def test_full_run_c1_fp32_and_fp64_vs_oracle(hg, orc, tmp_path):
+    """64^3 particles, 64^3 grid, box 128 Mpc/h, z=200 -> 0 in 60 steps (the reference's
+    CPU-runnable configuration): GPU FP32 engine and GPU FP64 plain path against the
+    FP64 oracle from identical initial conditions."""
+    case("full run 64^3 x 60")
+    ng, n = 64, 60
+    p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001, pk_bins=32)
+    hdr, vals = ipk_for(hg, p, str(tmp_path))
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    a = 1 / (1 + p.z_ini)
+    D, _ = hg.growth(p, a)
+    _, dD = hg.growth(p, a - ts.deltaT / 2)
+    pos, vel = orc.ic_f64(ng, ng, 128.0, 21082001, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
+    co = hg.schedule_array(p, n, merge=False)
+    po, vo, _ = orc.run_f64(ng, ng, co, pos, vel)
+    q_o = orc.cic_f64(ng, po)
+    k_o, pk_o, cnt_o = orc.power_grid(ng, 128.0, 0, 0, 32, q_o)
+    # FP32 engine, merged drifts
+    sched = hg.schedule(p, n, merge=True)
+    cell, off = orc.split_positions(ng, pos)
+    with hg.MemoryManager(p) as m:
+        m.set_particles_split(cell, off, vel.astype(np.float32))
+        for c in sched:
+            m.step(c)
+        c32, o32, v32 = m.get_particles_split()
+        k, pk, cnt = m.power_spectrum(0)
+    x32 = c32 + o32.astype(np.float64)
+    d = (x32 - po + ng / 2) % ng - ng / 2
+    rms = float(np.sqrt((d ** 2).mean()))
+    kny = np.pi * ng / 128.0
+    sel = (cnt_o > 0) & (k_o < kny / 2)
+    perr = float(np.abs(pk[sel] / pk_o[sel] - 1).max())
+    print(f"[result] fp32 rms displacement {rms:.3e} cells, max rel P(k) err {perr:.3e}", flush=True)
+    assert np.array_equal(cnt, cnt_o)
+    assert rms < 2e-3 and perr < 1e-3
+    # FP64 plain path
+    p64 = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001, pk_bins=32, precision=1)
+    with hg.MemoryManager(p64) as m:
+        m.set_particles_f64(pos, vel)
+        for c in hg.schedule(p64, n, merge=False):
+            m.step(c)
+        x64, v64 = m.get_particles_f64()
+    d = (x64 - po + ng / 2) % ng - ng / 2
+    assert np.sqrt((d ** 2).mean()) < 1e-9
+
+
+def test_driver_end_to_end(hg, tmp_path):
+    """haccgpm <params> writes the reference's outputs: <prefix>.params, P(k) text files
+    (k,P,count per line) for ini/steps/fin, raw float3 snapshots."""
+    case("driver 32^3 x 10")
+    import subprocess
+    p = hg.make_params(32, 32, 64.0, 50.0, 0.0, 10)
+    ipk = tmp_path / "eh.ipk"
+    hg.write_ipk_eh(p, str(ipk), 20.0, 20001)
+    prm = tmp_path / "run.params"
+    prm.write_text("\n".join([
+        "OMEGA_CDM 0.261", "DEUT 0.0226576", "OMEGA_NU 0.0", "HUBBLE 0.68", "SS8 0.81", "NS 0.96",
+        "W_DE -1.0", "WA_DE 0.0", "T_CMB 2.726", "N_EFF_MASSLESS 3.04", "N_EFF_MASSIVE 0.0",
+        f"OUTPUT_BASE_NAME {tmp_path}/out", f"IPK {ipk}", "Z_IN 50", "Z_FIN 0", "N_STEPS 10",
+        "PK_FOLDS 1", "PK_BINS 32", "PK_DUMP 0,4", "PARTICLE_DUMP 4", "DUMP_INIT", "DUMP_FINAL",
+        "NG 32", "NP 32", "RL 64", "SEED 5", "BLOCK_SIZE 64", ""]))
+    r = subprocess.run([hg.DRIVER_PATH, str(prm)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    for name in ["out.params", "out.pk.ini.fold.0", "out.pk.ini.fold.1", "out.pk.0.fold.0", "out.pk.4.fold.1",
+                 "out.pk.fin.fold.0", "out.particles.ini", "out.particles.4", "out.particles.fin"]:
+        assert (tmp_path / name).exists(), name
+    rows = [l.split(",") for l in (tmp_path / "out.pk.fin.fold.0").read_text().splitlines()]
+    assert len(rows) > 5 and all(len(r) == 3 for r in rows)
+    ks = np.array([float(r[0]) for r in rows])
+    assert np.all(np.diff(ks) > 0) and all(int(r[2]) > 0 for r in rows)
+    snap = np.fromfile(tmp_path / "out.particles.fin", np.float32)
+    assert snap.size == 6 * 32 ** 3
+    pos = snap[: 3 * 32 ** 3]
+    assert pos.min() >= 0 and pos.max() < 32
