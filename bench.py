@@ -15,7 +15,7 @@ Also reported on the same line:
   fp64_plain   : the plain FP64 GPU path (reference-structured kernels in FP64) on the
                  same workload, and the FP32/FP64 speed-up;
   cpu_baseline : the reference algorithm restated on the host CPU (the oracle in the
-                 reference's FP32 rounding) on 64^3 particles x 60 steps, threads stated.
+                 reference's FP32 rounding) on 64^3 particles x 120 steps, threads stated.
 """
 import argparse
 import json
@@ -91,10 +91,11 @@ def run_engine(hg, p, steps, warmup, timing=True):
 
 def cpu_baseline(hg, threads):
     """Reference algorithm on the host: the oracle in the reference's FP32 rounding
-    (float3 positions, FP32 CIC, reference calcGreens), 64^3 particles x 60 steps."""
+    (float3 positions, FP32 CIC, reference calcGreens), 64^3 particles, z=200 -> 0 in
+    120 steps (60 steps would make the reference's first half step a < 0)."""
     from oracle import oracle as orc
     orc.set_threads(threads)
-    ng, n = 64, 60
+    ng, n = 64, 120
     p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001)
     ts = hg.Timestepper(p)
     ts.setInitialZ(p.z_ini)

@@ -470,12 +470,12 @@ int hgpm_set_particles(hgpm_ctx* c, const float* pos, const float* vel) {
     const size_t n3 = 3 * (size_t)c->N;
     if (c->prec == HGPM_FP32) {
         float* tmp = nullptr;
-        int rc = dalloc(c, &tmp, sizeof(float) * n3, "staging");
+        int rc = dalloc(c, &tmp, sizeof(float) * 2 * n3, "staging");
         if (rc) return rc;
-        c->bytes -= sizeof(float) * n3;
+        c->bytes -= sizeof(float) * 2 * n3;
         HIPCHK(c, hipMemcpyAsync(tmp, pos, sizeof(float) * n3, hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(c->vel, vel, sizeof(float) * n3, hipMemcpyHostToDevice, c->st));
-        launch_split_from_float3(tmp, c->pos, c->N, c->ng, c->st);
+        HIPCHK(c, hipMemcpyAsync(tmp + n3, vel, sizeof(float) * n3, hipMemcpyHostToDevice, c->st));
+        launch_split_from_float3(tmp, tmp + n3, c->pos, c->vel, c->N, c->np, c->ng, c->st);
         LAUNCHCHK(c);
         HIPCHK(c, hipStreamSynchronize(c->st));
         (void)hipFree(tmp);
@@ -506,13 +506,13 @@ int hgpm_get_particles(hgpm_ctx* c, float* pos, float* vel) {
     const size_t n3 = 3 * (size_t)c->N;
     if (c->prec == HGPM_FP32) {
         float* tmp = nullptr;
-        rc = dalloc(c, &tmp, sizeof(float) * n3, "staging");
+        rc = dalloc(c, &tmp, sizeof(float) * 2 * n3, "staging");
         if (rc) return rc;
-        c->bytes -= sizeof(float) * n3;
-        launch_split_to_float3(c->pos, tmp, c->N, c->st);
+        c->bytes -= sizeof(float) * 2 * n3;
+        launch_split_to_float3(c->pos, c->vel, tmp, tmp + n3, c->N, c->np, c->st);
         LAUNCHCHK(c);
         HIPCHK(c, hipMemcpyAsync(pos, tmp, sizeof(float) * n3, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipMemcpyAsync(vel, c->vel, sizeof(float) * n3, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(vel, tmp + n3, sizeof(float) * n3, hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         (void)hipFree(tmp);
     } else {
@@ -539,9 +539,20 @@ int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off,
         }
         h[i] = PosSplitH{cell[3 * i], cell[3 * i + 1], cell[3 * i + 2], off[3 * i], off[3 * i + 1], off[3 * i + 2]};
     }
-    HIPCHK(c, hipMemcpyAsync(c->pos, h.data(), sizeof(PosSplitH) * n, hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->vel, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->st));
+    PosSplitH* sp = nullptr;
+    float* sv = nullptr;
+    int rc = dalloc(c, &sp, sizeof(PosSplitH) * n, "staging");
+    if (rc) return rc;
+    rc = dalloc(c, &sv, sizeof(float) * 3 * n, "staging");
+    if (rc) { (void)hipFree(sp); return rc; }
+    c->bytes -= sizeof(PosSplitH) * n + sizeof(float) * 3 * n;
+    HIPCHK(c, hipMemcpyAsync(sp, h.data(), sizeof(PosSplitH) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(sv, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->st));
+    launch_to_internal(sp, sv, c->pos, c->vel, c->N, c->np, c->st);
+    LAUNCHCHK(c);
     HIPCHK(c, hipStreamSynchronize(c->st));
+    (void)hipFree(sp);
+    (void)hipFree(sv);
     c->have_particles = true;
     c->have_density = c->have_force = false;
     return HGPM_OK;
@@ -554,9 +565,20 @@ int hgpm_get_particles_split(hgpm_ctx* c, int32_t* cell, float* off, float* vel)
     if (rc) return rc;
     const size_t n = (size_t)c->N;
     std::vector<PosSplitH> h(n);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->pos, sizeof(PosSplitH) * n, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(vel, c->vel, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->st));
+    PosSplitH* sp = nullptr;
+    float* sv = nullptr;
+    rc = dalloc(c, &sp, sizeof(PosSplitH) * n, "staging");
+    if (rc) return rc;
+    rc = dalloc(c, &sv, sizeof(float) * 3 * n, "staging");
+    if (rc) { (void)hipFree(sp); return rc; }
+    c->bytes -= sizeof(PosSplitH) * n + sizeof(float) * 3 * n;
+    launch_to_reference(c->pos, c->vel, sp, sv, c->N, c->np, c->st);
+    LAUNCHCHK(c);
+    HIPCHK(c, hipMemcpyAsync(h.data(), sp, sizeof(PosSplitH) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(vel, sv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    (void)hipFree(sp);
+    (void)hipFree(sv);
     for (size_t i = 0; i < n; i++) {
         cell[3 * i] = h[i].cx; cell[3 * i + 1] = h[i].cy; cell[3 * i + 2] = h[i].cz;
         off[3 * i] = h[i].fx; off[3 * i + 1] = h[i].fy; off[3 * i + 2] = h[i].fz;

@@ -47,3 +47,20 @@ struct __align__(8) PosSplit {
 };
 
 #define HGPM_WAVE 64
+
+// Internal particle order of the FP32 engine: Lagrangian tiles of TL^3 lattice points
+// (tile-major, x-major inside a tile), so that a contiguous range of particles is a
+// compact block in space.  ref_index() maps internal index j to the reference index
+// (qx*np + qy)*np + qz of placeParticles (place_particles_initializer.cu:49-65).
+#define HGPM_TL 16
+__host__ __device__ __forceinline__ int64_t ref_index(int64_t j, int np, int* qx, int* qy, int* qz) {
+    const int TL = HGPM_TL, TL3 = TL * TL * TL, nt = np / TL;
+    const int64_t t = j / TL3;
+    const int l = (int)(j - t * TL3);
+    const int tz = (int)(t % nt), ty = (int)((t / nt) % nt), tx = (int)(t / ((int64_t)nt * nt));
+    const int lz = l % TL, ly = (l / TL) % TL, lx = l / (TL * TL);
+    *qx = tx * TL + lx;
+    *qy = ty * TL + ly;
+    *qz = tz * TL + lz;
+    return ((int64_t)*qx * np + *qy) * np + *qz;
+}

@@ -67,6 +67,13 @@ int main(int argc, char** argv) {
     check(hgpm_ts_init(&p, &ts), ctx, "timestepper");
     hgpm_ts_set_initial_z(&p, &ts, p.z_ini);
     hgpm_ts_reverse_half_step(&p, &ts);
+    if (!(ts.aa > 0.0)) {
+        fprintf(stderr,
+                "haccgpm: a = %g after the initial half step back from Z_IN %g: N_STEPS %d is too small "
+                "(the reference's timestepper would produce NaN); use N_STEPS > %d\n",
+                ts.aa, p.z_ini, p.nsteps, (int)((1.0 - 1.0 / (1.0 + p.z_ini)) / (2.0 / (1.0 + p.z_ini))));
+        return 1;
+    }
 
     // initial conditions (initializer.cpp:65-166) from the .ipk table
     int nb = 0;

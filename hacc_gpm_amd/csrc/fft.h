@@ -21,17 +21,17 @@ template <int N> struct Plan;
     };
 HGPM_PLAN(16, 16)
 HGPM_PLAN(24, 3, 8)
-HGPM_PLAN(32, 8, 4)
+HGPM_PLAN(32, 4, 8)
 HGPM_PLAN(48, 3, 16)
 HGPM_PLAN(64, 8, 8)
-HGPM_PLAN(96, 3, 32)
+HGPM_PLAN(96, 3, 4, 8)
 HGPM_PLAN(128, 8, 16)
 HGPM_PLAN(192, 3, 8, 8)
 HGPM_PLAN(256, 16, 16)
 HGPM_PLAN(384, 3, 8, 16)
-HGPM_PLAN(512, 16, 32)
+HGPM_PLAN(512, 8, 8, 8)
 HGPM_PLAN(768, 3, 16, 16)
-HGPM_PLAN(1024, 32, 32)
+HGPM_PLAN(1024, 16, 8, 8)
 #undef HGPM_PLAN
 
 template <int N> __host__ __device__ constexpr int plan_L(int s) {

@@ -44,8 +44,15 @@ void launch_deposit(const PosSplitH* pos, int64_t np3, int ng, float mass, long 
                     unsigned long long* total, hipStream_t s);
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, hipStream_t s);
-void launch_split_from_float3(const float* pos3, PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
-void launch_split_to_float3(const PosSplitH* pos, float* pos3, int64_t np3, hipStream_t s);
+// conversions between the reference order (host arrays) and the internal tile order
+void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
+                              int ng, hipStream_t s);
+void launch_split_to_float3(const PosSplitH* pos, const float* vel, float* p3, float* v3, int64_t np3, int np,
+                            hipStream_t s);
+void launch_to_internal(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t np3, int np,
+                        hipStream_t s);
+void launch_to_reference(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t np3, int np,
+                         hipStream_t s);
 void launch_fold_split(PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
 void launch_force_aos(const float2* fxy, const float* fz, float* out, int64_t g3, hipStream_t s);
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s);

@@ -20,7 +20,8 @@
 
 namespace {
 constexpr int N = HGPM_FFT_N;
-constexpr int NT = 256;
+constexpr int NT = 256;   // rows passes
+constexpr int NTC = 512;  // column passes and x_solve: 16 elements per thread on 8192-element tiles
 
 // rows per row tile / columns per column tile (FP32, FP64)
 constexpr int rc32() { return N >= 768 ? 4 : (N >= 384 ? 8 : (N >= 192 ? 16 : (N >= 96 ? 32 : 64))); }
@@ -70,14 +71,15 @@ __global__ __launch_bounds__(NT) void k_rows_real_out(const float2* __restrict i
 // axis 1 (y): tile = (x, z-chunk), element (n=y, c) at x N^2 + n N + z0 + c
 // axis 0 (x): tile = (y, z-chunk), element (n=x, c) at n N^2 + y N + z0 + c
 template <typename V, int CC, int DIR>
-__global__ __launch_bounds__(NT) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis) {
+__global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis) {
     __shared__ V lds[lds_elems<N, CC>()];
-    const int tile = blockIdx.x;
+    // adjacent z-chunks of one row share 128-byte lines: keep them on one XCD's L2
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int outer = tile / (N / CC), zc = tile % (N / CC);
     const size_t NN = (size_t)N * N;
     const size_t base = axis == 1 ? (size_t)outer * NN + (size_t)zc * CC : (size_t)outer * N + (size_t)zc * CC;
     const size_t stride = axis == 1 ? (size_t)N : NN;
-    fft_tile<V, N, CC, NT, DIR, false, false>(
+    fft_tile<V, N, CC, NTC, DIR, false, false>(
         [&](int, int, int n, int c) { return d[base + n * stride + c]; },
         [&](int, int, int n, int c, V v) { d[base + n * stride + c] = v; }, lds, tw);
 }
@@ -90,35 +92,42 @@ __global__ __launch_bounds__(NT) void k_cols_c2c(V* __restrict d, const V* __res
 // src/pm_solver/cuda/calcGreens.hpp:11-26), G(0) = 0.  sk[m] = sin(2 pi m_s/N) with
 // the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274); hs[m] = sin^2(pi m/N).
 template <int CC>
-__global__ __launch_bounds__(NT) void k_x_solve(float2* __restrict a, float2* __restrict p,
-                                                const float2* __restrict twf, const float2* __restrict twr,
-                                                const float* __restrict sk, const float* __restrict hs,
-                                                float gcoef) {
+__global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* __restrict p,
+                                                 const float2* __restrict twf, const float2* __restrict twr,
+                                                 const float* __restrict sk, const float* __restrict hs,
+                                                 float gcoef) {
     __shared__ float2 lds[lds_elems<N, CC>()];
-    const int tile = blockIdx.x;
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int y = tile / (N / CC), zc = tile % (N / CC);
     const size_t NN = (size_t)N * N;
     const size_t base = (size_t)y * N + (size_t)zc * CC;
-    typedef LastShape<N, CC, NT> LS;
-    float2 P[LS::NB][LS::R], Q[LS::NB][LS::R];
+    typedef LastShape<N, CC, NTC> LS;
+    float2 R[LS::NB][LS::R]; // rho^ of this thread's elements, between the transforms
     const float hy = hs[y], sy = sk[y];
-    fft_tile<float2, N, CC, NT, -1, false, false>(
+    // g = (0.25/N^3) / sum_d sin^2(k_d/2) = -G  (0 at k = 0)
+    auto gfac = [&](int n, int z) {
+        const float ssum = (hs[n] + hy) + hs[z];
+        return (n | y | z) == 0 ? 0.0f : gcoef / ssum;
+    };
+    fft_tile<float2, N, CC, NTC, -1, false, false>(
         [&](int, int, int n, int c) { return a[base + n * NN + c]; },
-        [&](int b, int r, int n, int c, float2 v) {
-            const int z = zc * CC + c;
-            const float ssum = (hs[n] + hy) + hs[z];
-            const float g = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
-            const float sx = sk[n], sz = sk[z];
-            const float gx = g * v.x, gy = g * v.y;
-            P[b][r] = make_float2(-sy * gx - sx * gy, -sy * gy + sx * gx);
-            Q[b][r] = make_float2(-sz * gy, sz * gx);
+        [&](int b, int r, int, int, float2 v) { R[b][r] = v; }, lds, twf);
+    // P = Fx^ + i Fy^ = g (-sin k_y + i sin k_x) rho^
+    fft_tile<float2, N, CC, NTC, +1, true, false>(
+        [&](int b, int r, int n, int c) {
+            const float g = gfac(n, zc * CC + c), sx = sk[n];
+            const float gx = g * R[b][r].x, gy = g * R[b][r].y;
+            return make_float2(-sy * gx - sx * gy, -sy * gy + sx * gx);
         },
-        lds, twf);
-    fft_tile<float2, N, CC, NT, +1, true, false>(
-        [&](int b, int r, int, int) { return P[b][r]; },
         [&](int, int, int n, int c, float2 v) { p[base + n * NN + c] = v; }, lds, twr);
-    fft_tile<float2, N, CC, NT, +1, true, false>(
-        [&](int b, int r, int, int) { return Q[b][r]; },
+    // Q = Fz^ = i g sin k_z rho^
+    fft_tile<float2, N, CC, NTC, +1, true, false>(
+        [&](int b, int r, int n, int c) {
+            const int z = zc * CC + c;
+            const float g = gfac(n, z), sz = sk[z];
+            const float gx = g * R[b][r].x, gy = g * R[b][r].y;
+            return make_float2(-sz * gy, sz * gx);
+        },
         [&](int, int, int n, int c, float2 v) { a[base + n * NN + c] = v; }, lds, twr);
 }
 
@@ -131,8 +140,8 @@ void pass_impl(V* d, int axis, int dir, const V* tw, hipStream_t s) {
         else k_rows_c2c<V, RC, +1><<<nb, NT, 0, s>>>(d, tw);
     } else {
         unsigned nb = (unsigned)((size_t)N * (N / CC));
-        if (dir < 0) k_cols_c2c<V, CC, -1><<<nb, NT, 0, s>>>(d, tw, axis);
-        else k_cols_c2c<V, CC, +1><<<nb, NT, 0, s>>>(d, tw, axis);
+        if (dir < 0) k_cols_c2c<V, CC, -1><<<nb, NTC, 0, s>>>(d, tw, axis);
+        else k_cols_c2c<V, CC, +1><<<nb, NTC, 0, s>>>(d, tw, axis);
     }
 }
 
@@ -155,7 +164,7 @@ void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const f
              hipStream_t s) {
     unsigned nb = (unsigned)((size_t)N * (N / cc32()));
     const float gcoef = (float)(0.25 / ((double)N * N * N));
-    k_x_solve<cc32()><<<nb, NT, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
+    k_x_solve<cc32()><<<nb, NTC, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
 }
 void twiddles(double* fwd, double* rev) {
     make_twiddles<N, false>(fwd);

@@ -193,10 +193,12 @@ __global__ void k_disp_f64(const double2* __restrict amp, double2* __restrict ou
 // placeParticles for one axis: x = q ng/np + delta psi (FP64, then split), v = velMul psi.
 __global__ void k_place_split(const float2* __restrict sg, PosSplit* __restrict pos, float* __restrict vel, int np,
                               int ng, int dim, double delta, float vel_mul) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t G = (int64_t)np * np * np;
-    if (i >= G) return;
-    int q = dim == 0 ? (int)(i / ((int64_t)np * np)) : (dim == 1 ? (int)((i / np) % np) : (int)(i % np));
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; // internal (tile) order
+    const int64_t G = (int64_t)np * np * np;
+    if (j >= G) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    const int q = dim == 0 ? qx : (dim == 1 ? qy : qz);
     float s = (float)((double)sg[i].x * (1.0 / (double)G));
     double x = (double)q * ((double)ng / (double)np) + delta * (double)s;
     double fl = floor(x);
@@ -205,11 +207,11 @@ __global__ void k_place_split(const float2* __restrict sg, PosSplit* __restrict 
     if (f >= 1.0f) { f = 0.0f; c += 1; }
     c %= ng;
     if (c < 0) c += ng;
-    PosSplit& p = pos[i];
+    PosSplit& p = pos[j];
     if (dim == 0) { p.cx = c; p.fx = f; }
     else if (dim == 1) { p.cy = c; p.fy = f; }
     else { p.cz = c; p.fz = f; }
-    vel[3 * i + dim] = vel_mul * s;
+    vel[3 * j + dim] = vel_mul * s;
 }
 __global__ void k_place_f64(const double2* __restrict sg, double* __restrict pos, double* __restrict vel, int np,
                             int ng, int dim, double delta, double vel_mul) {

@@ -12,6 +12,8 @@
 #include "common.h"
 #include "internal.h"
 
+#include <climits>
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -63,34 +65,123 @@ __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, 
     return (long long)__float2ll_rn(w * 16777216.0f);
 }
 
-__global__ __launch_bounds__(BS) void k_deposit(const PosSplit* __restrict pos, int64_t n, int ng, float mass,
-                                                unsigned long long* __restrict rho,
-                                                unsigned long long* __restrict total) {
-    int64_t i = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    long long local = 0;
-    if (i < n) {
-        PosSplit p = pos[i];
-        const int x1 = p.cx + 1 == ng ? 0 : p.cx + 1;
-        const int y1 = p.cy + 1 == ng ? 0 : p.cy + 1;
-        const int z1 = p.cz + 1 == ng ? 0 : p.cz + 1;
-        const float gx[2] = {1.0f - p.fx, p.fx};
-        const float gy[2] = {1.0f - p.fy, p.fy};
-        const float gz[2] = {1.0f - p.fz, p.fz};
-        const int xs[2] = {p.cx, x1}, ys[2] = {p.cy, y1}, zs[2] = {p.cz, z1};
+// Tiled deposit.  A workgroup takes one Lagrangian tile (DEP_TILE consecutive particles
+// of the internal order, a compact block in space), finds the bounding box of their
+// cells relative to the first particle's cell (periodic-unwrapped), accumulates the 8
+// fixed-point CIC weights of every particle into an int64 window in LDS (ds_add_u64)
+// and adds the non-empty window cells to the global density with coalesced int64
+// atomics.  A tile whose window exceeds WMAX cells is processed in slabs (x, then y):
+// each pass accumulates the corners that fall inside the slab.  Integer sums are
+// order-free, so the result is bit-identical to any sequential order.
+constexpr int DEP_NT = 512;
+constexpr int DEP_PPT = 8;
+constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
+constexpr int WMAX = 9216;                 // 72 KB of int64 per workgroup
+static_assert(DEP_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
+
+__device__ __forceinline__ int wrap_rel(int d, int ng) {
+    const int h = ng >> 1;
+    if (d >= h) d -= ng;
+    else if (d < -h) d += ng;
+    return d;
+}
+__device__ __forceinline__ int wrap_cell(int v, int ng) {
+    v %= ng;
+    return v < 0 ? v + ng : v;
+}
+
+__global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __restrict pos, int64_t n, int ng,
+                                                          float mass, unsigned long long* __restrict rho,
+                                                          unsigned long long* __restrict total) {
+    __shared__ unsigned long long win[WMAX];
+    __shared__ int red[6][DEP_NT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * DEP_TILE;
+    const PosSplit r0 = pos[p0];
+    int rx[DEP_PPT], ry[DEP_PPT], rz[DEP_PPT];
+    float fx[DEP_PPT], fy[DEP_PPT], fz[DEP_PPT];
+    bool ok[DEP_PPT];
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-        for (int x = 0; x < 2; x++)
-#pragma unroll
-            for (int y = 0; y < 2; y++)
-#pragma unroll
-                for (int z = 0; z < 2; z++) {
-                    long long q = fixed_weight(gx[x], gy[y], gz[z], mass);
-                    local += q;
-                    atomicAdd(&rho[gidx(ng, xs[x], ys[y], zs[z])], (unsigned long long)q);
-                }
+    for (int k = 0; k < DEP_PPT; k++) {
+        const int64_t j = p0 + tid + k * DEP_NT;
+        ok[k] = j < n;
+        PosSplit p = ok[k] ? pos[j] : r0;
+        rx[k] = wrap_rel(p.cx - r0.cx, ng);
+        ry[k] = wrap_rel(p.cy - r0.cy, ng);
+        rz[k] = wrap_rel(p.cz - r0.cz, ng);
+        fx[k] = p.fx; fy[k] = p.fy; fz[k] = p.fz;
+        mn[0] = min(mn[0], rx[k]); mx[0] = max(mx[0], rx[k]);
+        mn[1] = min(mn[1], ry[k]); mx[1] = max(mx[1], ry[k]);
+        mn[2] = min(mn[2], rz[k]); mx[2] = max(mx[2], rz[k]);
     }
-    // wave reduction of the deposited total (integer: order-free)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[d] = min(mn[d], __shfl_xor(mn[d], off, 64));
+            mx[d] = max(mx[d], __shfl_xor(mx[d], off, 64));
+        }
+        if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
+    }
+    __syncthreads();
+    int lo[3], e[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int a = red[d][0], b = red[3 + d][0];
+        for (int w = 1; w < DEP_NT / 64; w++) { a = min(a, red[d][w]); b = max(b, red[3 + d][w]); }
+        lo[d] = a;
+        e[d] = b - a + 2; // + the upper CIC neighbour
+    }
+    const int ty = min(e[1], WMAX / e[2]);
+    const int tx = min(e[0], WMAX / (ty * e[2]));
+    long long local = 0;
+    for (int sx0 = 0; sx0 < e[0]; sx0 += tx) {
+        const int nx = min(tx, e[0] - sx0);
+        for (int sy0 = 0; sy0 < e[1]; sy0 += ty) {
+            const int ny = min(ty, e[1] - sy0);
+            const int vol = nx * ny * e[2];
+            for (int i = tid; i < vol; i += DEP_NT) win[i] = 0ull;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < DEP_PPT; k++) {
+                if (!ok[k]) continue;
+                const int bx = rx[k] - lo[0] - sx0, by = ry[k] - lo[1] - sy0, bz = rz[k] - lo[2];
+                const float gx[2] = {1.0f - fx[k], fx[k]};
+                const float gy[2] = {1.0f - fy[k], fy[k]};
+                const float gz[2] = {1.0f - fz[k], fz[k]};
+#pragma unroll
+                for (int x = 0; x < 2; x++) {
+                    const int lx = bx + x;
+                    if (lx < 0 || lx >= nx) continue;
+#pragma unroll
+                    for (int y = 0; y < 2; y++) {
+                        const int ly = by + y;
+                        if (ly < 0 || ly >= ny) continue;
+#pragma unroll
+                        for (int z = 0; z < 2; z++) {
+                            const long long q = fixed_weight(gx[x], gy[y], gz[z], mass);
+                            local += q;
+                            atomicAdd(&win[(lx * ny + ly) * e[2] + bz + z], (unsigned long long)q);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int nyz = ny * e[2];
+            for (int i = tid; i < vol; i += DEP_NT) {
+                const unsigned long long v = win[i];
+                if (!v) continue;
+                const int ix = i / nyz, rem = i - ix * nyz, iy = rem / e[2], iz = rem - iy * e[2];
+                const int gx = wrap_cell(r0.cx + lo[0] + sx0 + ix, ng);
+                const int gy = wrap_cell(r0.cy + lo[1] + sy0 + iy, ng);
+                const int gz = wrap_cell(r0.cz + lo[2] + iz, ng);
+                atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
+            }
+            __syncthreads();
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
-    if ((threadIdx.x & 63) == 0 && local != 0) atomicAdd(total, (unsigned long long)local);
+    if (lane == 0 && local != 0) atomicAdd(total, (unsigned long long)local);
 }
 
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
@@ -138,28 +229,56 @@ __global__ __launch_bounds__(BS) void k_gather_kick(PosSplit* __restrict pos, fl
     }
 }
 
-// float3 grid-unit positions -> (cell, offset): c = floor(x), f = x - c exact in FP32.
-__global__ void k_split_from_float3(const float* __restrict p3, PosSplit* __restrict pos, int64_t n, int ng) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// float3 grid-unit positions in reference order -> (cell, offset) in internal order:
+// c = floor(x), f = x - c exact in FP32.
+__global__ void k_split_from_float3(const float* __restrict p3, const float* __restrict v3, PosSplit* __restrict pos,
+                                    float* __restrict vel, int64_t n, int np, int ng) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
     PosSplit p;
-    float x = p3[3 * i], y = p3[3 * i + 1], z = p3[3 * i + 2];
     p.cx = 0; p.cy = 0; p.cz = 0;
-    p.fx = x; p.fy = y; p.fz = z;
+    p.fx = p3[3 * i]; p.fy = p3[3 * i + 1]; p.fz = p3[3 * i + 2];
     carry_wrap(p.cx, p.fx, ng);
     carry_wrap(p.cy, p.fy, ng);
     carry_wrap(p.cz, p.fz, ng);
-    pos[i] = p;
+    pos[j] = p;
+    vel[3 * j] = v3[3 * i]; vel[3 * j + 1] = v3[3 * i + 1]; vel[3 * j + 2] = v3[3 * i + 2];
 }
 
-// (cell, offset) -> float3 grid units: float((double)c + f) (snapshot layout).
-__global__ void k_split_to_float3(const PosSplit* __restrict pos, float* __restrict p3, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    PosSplit p = pos[i];
+// internal order -> float3 grid units float((double)c + f) in reference order (snapshot layout).
+__global__ void k_split_to_float3(const PosSplit* __restrict pos, const float* __restrict vel, float* __restrict p3,
+                                  float* __restrict v3, int64_t n, int np) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    PosSplit p = pos[j];
     p3[3 * i] = (float)((double)p.cx + (double)p.fx);
     p3[3 * i + 1] = (float)((double)p.cy + (double)p.fy);
     p3[3 * i + 2] = (float)((double)p.cz + (double)p.fz);
+    v3[3 * i] = vel[3 * j]; v3[3 * i + 1] = vel[3 * j + 1]; v3[3 * i + 2] = vel[3 * j + 2];
+}
+
+// reference order <-> internal order for the split layout
+__global__ void k_to_internal(const PosSplit* __restrict src, const float* __restrict vsrc, PosSplit* __restrict dst,
+                              float* __restrict vdst, int64_t n, int np) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    dst[j] = src[i];
+    vdst[3 * j] = vsrc[3 * i]; vdst[3 * j + 1] = vsrc[3 * i + 1]; vdst[3 * j + 2] = vsrc[3 * i + 2];
+}
+__global__ void k_to_reference(const PosSplit* __restrict src, const float* __restrict vsrc, PosSplit* __restrict dst,
+                               float* __restrict vdst, int64_t n, int np) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    dst[i] = src[j];
+    vdst[3 * i] = vsrc[3 * j]; vdst[3 * i + 1] = vsrc[3 * j + 1]; vdst[3 * i + 2] = vsrc[3 * j + 2];
 }
 
 // foldParticles (src/power_spectrum/cuda/fold_particles_power.cu:3-27): x -> 2 (x mod ng/2),
@@ -220,8 +339,8 @@ void launch_deposit(const PosSplitH* pos, int64_t n, int ng, float mass, long lo
                     unsigned long long* total, hipStream_t s) {
     // rho and total must be zero (the caller clears them with launch_zero_u64)
     if (!n) return;
-    k_deposit<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho,
-                                                total);
+    unsigned nb = (unsigned)((n + DEP_TILE - 1) / DEP_TILE);
+    k_deposit_tiled<<<nb, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho, total);
 }
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, hipStream_t s) {
@@ -229,13 +348,25 @@ void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const flo
     k_gather_kick<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift,
                                                     pf);
 }
-void launch_split_from_float3(const float* p3, PosSplitH* pos, int64_t n, int ng, hipStream_t s) {
+void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
+                              int ng, hipStream_t s) {
     if (!n) return;
-    k_split_from_float3<<<grid_blocks(n, BS), BS, 0, s>>>(p3, (PosSplit*)pos, n, ng);
+    k_split_from_float3<<<grid_blocks(n, BS), BS, 0, s>>>(p3, v3, (PosSplit*)pos, vel, n, np, ng);
 }
-void launch_split_to_float3(const PosSplitH* pos, float* p3, int64_t n, hipStream_t s) {
+void launch_split_to_float3(const PosSplitH* pos, const float* vel, float* p3, float* v3, int64_t n, int np,
+                            hipStream_t s) {
     if (!n) return;
-    k_split_to_float3<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)pos, p3, n);
+    k_split_to_float3<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)pos, vel, p3, v3, n, np);
+}
+void launch_to_internal(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t n, int np,
+                        hipStream_t s) {
+    if (!n) return;
+    k_to_internal<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)src, vsrc, (PosSplit*)dst, vdst, n, np);
+}
+void launch_to_reference(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t n, int np,
+                         hipStream_t s) {
+    if (!n) return;
+    k_to_reference<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)src, vsrc, (PosSplit*)dst, vdst, n, np);
 }
 void launch_fold_split(PosSplitH* pos, int64_t n, int ng, hipStream_t s) {
     if (!n) return;

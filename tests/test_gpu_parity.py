@@ -257,11 +257,12 @@ def test_initial_conditions(hg, orc, tmp_path, prec):
 
 # -------------------------------------------------------------- full run ----
 def test_full_run_c1_fp32_and_fp64_vs_oracle(hg, orc, tmp_path):
-    """64^3 particles, 64^3 grid, box 128 Mpc/h, z=200 -> 0 in 60 steps (the reference's
-    CPU-runnable configuration): GPU FP32 engine and GPU FP64 plain path against the
+    """64^3 particles, 64^3 grid, box 128 Mpc/h, z=200 -> 0 (the reference's CPU-runnable
+    configuration) in 120 a-uniform steps -- with 60 steps the reference's initial
+    reverseHalfStep makes a < 0 -- : GPU FP32 engine and GPU FP64 plain path against the
     FP64 oracle from identical initial conditions."""
-    case("full run 64^3 x 60")
-    ng, n = 64, 60
+    case("full run 64^3 x 120")
+    ng, n = 64, 120
     p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001, pk_bins=32)
     hdr, vals = ipk_for(hg, p, str(tmp_path))
     ts = hg.Timestepper(p)
@@ -309,14 +310,14 @@ def test_driver_end_to_end(hg, tmp_path):
     (k,P,count per line) for ini/steps/fin, raw float3 snapshots."""
     case("driver 32^3 x 10")
     import subprocess
-    p = hg.make_params(32, 32, 64.0, 50.0, 0.0, 10)
+    p = hg.make_params(32, 32, 64.0, 50.0, 0.0, 100)
     ipk = tmp_path / "eh.ipk"
     hg.write_ipk_eh(p, str(ipk), 20.0, 20001)
     prm = tmp_path / "run.params"
     prm.write_text("\n".join([
         "OMEGA_CDM 0.261", "DEUT 0.0226576", "OMEGA_NU 0.0", "HUBBLE 0.68", "SS8 0.81", "NS 0.96",
         "W_DE -1.0", "WA_DE 0.0", "T_CMB 2.726", "N_EFF_MASSLESS 3.04", "N_EFF_MASSIVE 0.0",
-        f"OUTPUT_BASE_NAME {tmp_path}/out", f"IPK {ipk}", "Z_IN 50", "Z_FIN 0", "N_STEPS 10",
+        f"OUTPUT_BASE_NAME {tmp_path}/out", f"IPK {ipk}", "Z_IN 50", "Z_FIN 0", "N_STEPS 100", "LAST_STEP 10",
         "PK_FOLDS 1", "PK_BINS 32", "PK_DUMP 0,4", "PARTICLE_DUMP 4", "DUMP_INIT", "DUMP_FINAL",
         "NG 32", "NP 32", "RL 64", "SEED 5", "BLOCK_SIZE 64", ""]))
     r = subprocess.run([hg.DRIVER_PATH, str(prm)], capture_output=True, text=True, timeout=120)
