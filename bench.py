@@ -68,6 +68,7 @@ def run_engine(hg, p, steps, warmup, timing=True):
     n_total = warmup + steps
     sched = hg.schedule(p, n_total, merge=True)
     with hg.MemoryManager(p) as m:
+        tune = m.autotune(3) if p.precision == hg.FP32 else None
         t0 = time.perf_counter()
         hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
         m.synchronize()
@@ -86,7 +87,7 @@ def run_engine(hg, p, steps, warmup, timing=True):
         el = time.perf_counter() - t0
         tim = m.timings() if timing else {}
         dev_bytes = m.device_bytes()
-    return el, tim, t_ic, dev_bytes
+    return el, tim, t_ic, dev_bytes, tune
 
 
 def cpu_baseline(hg, threads):
@@ -135,7 +136,7 @@ def main():
     p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221)
     Np = ng ** 3
     G = ng ** 3
-    el, tim, t_ic, dev_bytes = run_engine(hg, p, args.steps, args.warmup)
+    el, tim, t_ic, dev_bytes, tune = run_engine(hg, p, args.steps, args.warmup)
     value = Np * args.steps / el
     ms_step = el * 1e3 / args.steps
     # dominant kernel region
@@ -172,13 +173,14 @@ def main():
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / HBM_PEAK},
         "kernels": breakdown,
+        "autotune": tune,
         "ic_seconds": t_ic,
         "device_GB": dev_bytes / 1e9,
     }
     if not args.no_fp64:
         p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221, precision=hg.FP64)
         try:
-            el64, _, _, b64 = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
+            el64, _, _, b64, _ = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
             v64 = Np * args.fp64_steps / el64
             out["fp64_plain"] = {"value": v64, "ms_per_step": el64 * 1e3 / args.fp64_steps,
                                  "steps": args.fp64_steps, "device_GB": b64 / 1e9}

@@ -132,6 +132,7 @@ def lib():
             "hgpm_reset_timings": (c_int, [vp]),
             "hgpm_set_tunable": (c_int, [vp, c_char_p, c_int]),
             "hgpm_get_tunable": (c_int, [vp, c_char_p, P(c_int)]),
+            "hgpm_autotune": (c_int, [vp, c_int, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -406,6 +407,25 @@ class MemoryManager:
 
     def write_output(self, fname):
         self._c(lib().hgpm_write_output(self.h, str(fname).encode()))
+
+    # tuning
+    def set_tunable(self, key, value):
+        self._c(lib().hgpm_set_tunable(self.h, key.encode(), int(value)))
+
+    def get_tunable(self, key):
+        v = ctypes.c_int()
+        self._c(lib().hgpm_get_tunable(self.h, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, reps=3):
+        """Times the FFT variants on this grid and keeps the fastest; returns the
+        per-variant milliseconds (column pass, fused x-solve)."""
+        mc = np.zeros(4)
+        mx = np.zeros(4)
+        self._c(lib().hgpm_autotune(self.h, int(reps), _ptr(mc), _ptr(mx)))
+        n = self.get_tunable("fft_variants")
+        return {"cols_ms": mc[:n].tolist(), "xsolve_ms": mx[:n].tolist(),
+                "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve")}
 
     # instrumentation
     def set_timing(self, on=True):

@@ -65,8 +65,9 @@ struct hgpm_ctx {
     std::vector<KStat> stats;
     std::vector<PendingEv> pending;
     std::vector<hipEvent_t> evpool;
-    // tunables (reserved for the autotuner)
-    std::map<std::string, int> tun;
+    // tunables: column-pass and x_solve variants of the FFT (see k_fft.hip), set by
+    // hgpm_set_tunable or hgpm_autotune
+    int var_cols = 2, var_xs = 2;
 };
 
 namespace {
@@ -180,9 +181,10 @@ int upload_twiddles(hgpm_ctx* c, const FftOps* ops, V** f, V** r) {
 // 3-D transform of an n^3 grid through the generic passes (z, y, x).
 int fft3d_f32(hgpm_ctx* c, const FftOps* ops, float2* d, const float2* tw, int dir) {
     Timed t(c, "fft3d_f32");
-    ops->pass_f32(d, 2, dir, tw, c->st);
-    ops->pass_f32(d, 1, dir, tw, c->st);
-    ops->pass_f32(d, 0, dir, tw, c->st);
+    const int v = ops == c->fft ? c->var_cols : 2;
+    ops->pass_f32(d, 2, dir, tw, v, c->st);
+    ops->pass_f32(d, 1, dir, tw, v, c->st);
+    ops->pass_f32(d, 0, dir, tw, v, c->st);
     LAUNCHCHK(c);
     return HGPM_OK;
 }
@@ -221,20 +223,20 @@ int f32_solve(hgpm_ctx* c) {
     }
     {
         Timed t(c, "fft_fwd_y");
-        f->pass_f32(c->A, 1, -1, c->twf, c->st);
+        f->pass_f32(c->A, 1, -1, c->twf, c->var_cols, c->st);
     }
     {
         Timed t(c, "fft_x_solve");
-        f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, c->st);
+        f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->st);
     }
     {
         Timed t(c, "fft_inv_y");
-        f->pass_f32(c->B, 1, +1, c->twf, c->st);
-        f->pass_f32(c->A, 1, +1, c->twf, c->st);
+        f->pass_f32(c->B, 1, +1, c->twf, c->var_cols, c->st);
+        f->pass_f32(c->A, 1, +1, c->twf, c->var_cols, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fxy");
-        f->pass_f32(c->B, 2, +1, c->twf, c->st);
+        f->pass_f32(c->B, 2, +1, c->twf, c->var_cols, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fz");
@@ -868,8 +870,8 @@ int hgpm_power_spectrum_grid(hgpm_ctx* c, int fold, double* k, double* pk, int64
         }
         {
             Timed t(c, "pk_fwd_yx");
-            c->fft->pass_f32(c->A, 1, -1, c->twf, c->st);
-            c->fft->pass_f32(c->A, 0, -1, c->twf, c->st);
+            c->fft->pass_f32(c->A, 1, -1, c->twf, c->var_cols, c->st);
+            c->fft->pass_f32(c->A, 0, -1, c->twf, c->var_cols, c->st);
         }
         LAUNCHCHK(c);
     } else {
@@ -995,11 +997,63 @@ int hgpm_get_timings(hgpm_ctx* c, char* names, int names_len, double* total_ms, 
 }
 int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
     if (!c || !key) return HGPM_ERR_ARG;
-    return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key + " (" + std::to_string(value) + ")");
+    const int nv = c->fft ? c->fft->valid_variants : 0;
+    if (!strcmp(key, "fft_cols") || !strcmp(key, "fft_xsolve")) {
+        if (value < 0 || value >= nv) return fail(c, HGPM_ERR_ARG, std::string(key) + " variant out of range");
+        (!strcmp(key, "fft_cols") ? c->var_cols : c->var_xs) = value;
+        return HGPM_OK;
+    }
+    return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
 }
 int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     if (!c || !key || !value) return HGPM_ERR_ARG;
+    if (!strcmp(key, "fft_cols")) { *value = c->var_cols; return HGPM_OK; }
+    if (!strcmp(key, "fft_xsolve")) { *value = c->var_xs; return HGPM_OK; }
+    if (!strcmp(key, "fft_variants")) { *value = c->fft ? c->fft->valid_variants : 0; return HGPM_OK; }
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
+}
+
+// Times every column-pass and x_solve variant of this grid size on the context's own
+// grids and keeps the fastest.  Overwrites the density / force grids.
+int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
+    if (!c) return HGPM_ERR_ARG;
+    if (c->prec != HGPM_FP32) return HGPM_OK; // the FP64 plain path has one shape
+    if (reps < 1) reps = 3;
+    const FftOps* f = c->fft;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    launch_zero_u64((unsigned long long*)c->A, (size_t)c->G, c->st);
+    launch_zero_u64((unsigned long long*)c->B, (size_t)c->G, c->st);
+    double best_c = 1e30, best_x = 1e30;
+    for (int v = 0; v < f->valid_variants; v++) {
+        for (int kind = 0; kind < 2; kind++) {
+            auto run = [&]() {
+                if (kind == 0) f->pass_f32(c->A, 1, -1, c->twf, v, c->st);
+                else f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, v, c->st);
+            };
+            run();
+            HIPCHK(c, hipEventRecord(a, c->st));
+            for (int r = 0; r < reps; r++) run();
+            HIPCHK(c, hipEventRecord(b, c->st));
+            HIPCHK(c, hipEventSynchronize(b));
+            LAUNCHCHK(c);
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+            double per = ms / reps;
+            if (kind == 0) {
+                if (ms_cols) ms_cols[v] = per;
+                if (per < best_c) { best_c = per; c->var_cols = v; }
+            } else {
+                if (ms_xsolve) ms_xsolve[v] = per;
+                if (per < best_x) { best_x = per; c->var_xs = v; }
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->have_density = c->have_force = false;
+    return HGPM_OK;
 }
 
 } // extern "C"

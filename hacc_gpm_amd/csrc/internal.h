@@ -10,14 +10,17 @@ namespace hgpm {
 struct FftOps {
     int n;
     int stages;
+    int n_variants;      // column-pass / x_solve variants compiled (ids 0..n_variants-1)
+    int valid_variants;  // variants usable for this length
     void (*twiddles)(double* fwd, double* rev); // stages*n complex each
-    void (*pass_f32)(float2* d, int axis, int dir, const float2* tw, hipStream_t s);
+    // axis 2 = rows (z), 1 = y, 0 = x; variant selects the column-pass shape
+    void (*pass_f32)(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s);
     void (*pass_f64)(double2* d, int axis, int dir, const double2* tw, hipStream_t s);
     void (*fwd_z_density)(const long long* rho, const unsigned long long* total, float2* out,
                           const float2* tw, hipStream_t s);
     void (*inv_z_real)(const float2* in, float* out, const float2* tw, hipStream_t s);
     void (*x_solve)(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk,
-                    const float* hs, hipStream_t s);
+                    const float* hs, int variant, hipStream_t s);
 };
 
 #define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)

@@ -21,7 +21,6 @@
 namespace {
 constexpr int N = HGPM_FFT_N;
 constexpr int NT = 256;   // rows passes
-constexpr int NTC = 512;  // column passes and x_solve: 16 elements per thread on 8192-element tiles
 
 // rows per row tile / columns per column tile (FP32, FP64)
 constexpr int rc32() { return N >= 768 ? 4 : (N >= 384 ? 8 : (N >= 192 ? 16 : (N >= 96 ? 32 : 64))); }
@@ -70,7 +69,7 @@ __global__ __launch_bounds__(NT) void k_rows_real_out(const float2* __restrict i
 // ------------------------------------------------------------- cols ----
 // axis 1 (y): tile = (x, z-chunk), element (n=y, c) at x N^2 + n N + z0 + c
 // axis 0 (x): tile = (y, z-chunk), element (n=x, c) at n N^2 + y N + z0 + c
-template <typename V, int CC, int DIR>
+template <typename V, int CC, int NTC, int DIR>
 __global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis) {
     __shared__ V lds[lds_elems<N, CC>()];
     // adjacent z-chunks of one row share 128-byte lines: keep them on one XCD's L2
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __re
 // (kspace_solve_gradient, src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens,
 // src/pm_solver/cuda/calcGreens.hpp:11-26), G(0) = 0.  sk[m] = sin(2 pi m_s/N) with
 // the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274); hs[m] = sin^2(pi m/N).
-template <int CC>
+template <int CC, int NTC>
 __global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* __restrict p,
                                                  const float2* __restrict twf, const float2* __restrict twr,
                                                  const float* __restrict sk, const float* __restrict hs,
@@ -102,54 +101,78 @@ __global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* _
     const size_t NN = (size_t)N * N;
     const size_t base = (size_t)y * N + (size_t)zc * CC;
     typedef LastShape<N, CC, NTC> LS;
+    typedef StageShape<N, CC, NTC, LS::R> SS;
     float2 R[LS::NB][LS::R]; // rho^ of this thread's elements, between the transforms
-    const float hy = hs[y], sy = sk[y];
-    // g = (0.25/N^3) / sum_d sin^2(k_d/2) = -G  (0 at k = 0)
-    auto gfac = [&](int n, int z) {
-        const float ssum = (hs[n] + hy) + hs[z];
-        return (n | y | z) == 0 ? 0.0f : gcoef / ssum;
-    };
+    // The factors this thread needs, loaded before the forward transform so their
+    // latency hides behind it: its column c (fixed: NTC is a multiple of CC) gives z,
+    // its butterflies give the x modes n = j + r*N/R.
+    const int c0 = threadIdx.x % CC;
+    const int z = zc * CC + c0;
+    const float sy = sk[y], sz = sk[z];
+    float G[LS::NB][LS::R], SX[LS::NB][LS::R];
+#pragma unroll
+    for (int b = 0; b < LS::NB; b++)
+#pragma unroll
+        for (int r = 0; r < LS::R; r++) {
+            const int w = threadIdx.x + b * NTC;
+            const int n = (SS::TOT % NTC == 0 || w < SS::TOT) ? (w / CC + r * SS::NBF) : 0;
+            // g = (0.25/N^3) / sum_d sin^2(k_d/2) = -G (0 at k = 0)
+            const float ssum = (hs[n] + hs[y]) + hs[z];
+            G[b][r] = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
+            SX[b][r] = sk[n];
+        }
     fft_tile<float2, N, CC, NTC, -1, false, false>(
         [&](int, int, int n, int c) { return a[base + n * NN + c]; },
         [&](int b, int r, int, int, float2 v) { R[b][r] = v; }, lds, twf);
     // P = Fx^ + i Fy^ = g (-sin k_y + i sin k_x) rho^
     fft_tile<float2, N, CC, NTC, +1, true, false>(
-        [&](int b, int r, int n, int c) {
-            const float g = gfac(n, zc * CC + c), sx = sk[n];
-            const float gx = g * R[b][r].x, gy = g * R[b][r].y;
+        [&](int b, int r, int, int) {
+            const float gx = G[b][r] * R[b][r].x, gy = G[b][r] * R[b][r].y, sx = SX[b][r];
             return make_float2(-sy * gx - sx * gy, -sy * gy + sx * gx);
         },
         [&](int, int, int n, int c, float2 v) { p[base + n * NN + c] = v; }, lds, twr);
     // Q = Fz^ = i g sin k_z rho^
     fft_tile<float2, N, CC, NTC, +1, true, false>(
-        [&](int b, int r, int n, int c) {
-            const int z = zc * CC + c;
-            const float g = gfac(n, z), sz = sk[z];
-            const float gx = g * R[b][r].x, gy = g * R[b][r].y;
+        [&](int b, int r, int, int) {
+            const float gx = G[b][r] * R[b][r].x, gy = G[b][r] * R[b][r].y;
             return make_float2(-sz * gy, sz * gx);
         },
         [&](int, int, int n, int c, float2 v) { a[base + n * NN + c] = v; }, lds, twr);
 }
 
 // --------------------------------------------------------- launchers ----
-template <typename V, int RC, int CC>
-void pass_impl(V* d, int axis, int dir, const V* tw, hipStream_t s) {
-    if (axis == 2) {
-        unsigned nb = (unsigned)((size_t)N * N / RC);
-        if (dir < 0) k_rows_c2c<V, RC, -1><<<nb, NT, 0, s>>>(d, tw);
-        else k_rows_c2c<V, RC, +1><<<nb, NT, 0, s>>>(d, tw);
-    } else {
-        unsigned nb = (unsigned)((size_t)N * (N / CC));
-        if (dir < 0) k_cols_c2c<V, CC, -1><<<nb, NTC, 0, s>>>(d, tw, axis);
-        else k_cols_c2c<V, CC, +1><<<nb, NTC, 0, s>>>(d, tw, axis);
+// Column-pass / x_solve variants (columns per tile, threads per workgroup):
+//   0: (4, 256)  1: (4, 512)  2: (8, 512)  3: (16, 512)
+// chosen per transform length by the autotuner (default below).
+template <typename V, int CC, int NTC>
+void cols_launch(V* d, int axis, int dir, const V* tw, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * (N / CC));
+    if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis);
+    else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis);
+}
+template <typename V, int RC>
+void rows_launch(V* d, int dir, const V* tw, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * N / RC);
+    if (dir < 0) k_rows_c2c<V, RC, -1><<<nb, NT, 0, s>>>(d, tw);
+    else k_rows_c2c<V, RC, +1><<<nb, NT, 0, s>>>(d, tw);
+}
+constexpr int cc_of(int v) { return v == 0 ? 4 : (v == 1 ? 4 : (v == 2 ? 8 : 16)); }
+constexpr int nt_of(int v) { return v == 0 ? 256 : 512; }
+constexpr bool variant_ok(int v) { return cc_of(v) <= N && (N % cc_of(v)) == 0 && (size_t)N * cc_of(v) * 8 <= 140000; }
+
+void pass_f32(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s) {
+    if (axis == 2) { rows_launch<float2, rc32()>(d, dir, tw, s); return; }
+    switch (variant) {
+    case 0: cols_launch<float2, cc_of(0), nt_of(0)>(d, axis, dir, tw, s); break;
+    case 1: cols_launch<float2, cc_of(1), nt_of(1)>(d, axis, dir, tw, s); break;
+    case 3: if constexpr (variant_ok(3)) { cols_launch<float2, cc_of(3), nt_of(3)>(d, axis, dir, tw, s); break; }
+            [[fallthrough]];
+    default: cols_launch<float2, cc_of(2), nt_of(2)>(d, axis, dir, tw, s); break;
     }
 }
-
-void pass_f32(float2* d, int axis, int dir, const float2* tw, hipStream_t s) {
-    pass_impl<float2, rc32(), cc32()>(d, axis, dir, tw, s);
-}
 void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
-    pass_impl<double2, rc64(), cc64()>(d, axis, dir, tw, s);
+    if (axis == 2) rows_launch<double2, rc64()>(d, dir, tw, s);
+    else cols_launch<double2, cc64(), 512>(d, axis, dir, tw, s);
 }
 void fwd_z_density(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
                    hipStream_t s) {
@@ -160,11 +183,22 @@ void inv_z_real(const float2* in, float* out, const float2* tw, hipStream_t s) {
     unsigned nb = (unsigned)((size_t)N * N / rc32());
     k_rows_real_out<rc32()><<<nb, NT, 0, s>>>(in, out, tw);
 }
-void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
-             hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * (N / cc32()));
+template <int CC, int NTC>
+void x_solve_launch(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
+                    hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * (N / CC));
     const float gcoef = (float)(0.25 / ((double)N * N * N));
-    k_x_solve<cc32()><<<nb, NTC, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
+    k_x_solve<CC, NTC><<<nb, NTC, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
+}
+void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
+             int variant, hipStream_t s) {
+    switch (variant) {
+    case 0: x_solve_launch<cc_of(0), nt_of(0)>(a, p, twf, twr, sk, hs, s); break;
+    case 1: x_solve_launch<cc_of(1), nt_of(1)>(a, p, twf, twr, sk, hs, s); break;
+    case 3: if constexpr (variant_ok(3)) { x_solve_launch<cc_of(3), nt_of(3)>(a, p, twf, twr, sk, hs, s); break; }
+            [[fallthrough]];
+    default: x_solve_launch<cc_of(2), nt_of(2)>(a, p, twf, twr, sk, hs, s); break;
+    }
 }
 void twiddles(double* fwd, double* rev) {
     make_twiddles<N, false>(fwd);
@@ -178,7 +212,7 @@ namespace hgpm {
 #define HGPM_CAT2(a, b) a##b
 #define HGPM_CAT(a, b) HGPM_CAT2(a, b)
 extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
-    N, Plan<N>::S, twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
+    N, Plan<N>::S, 4, variant_ok(3) ? 4 : 3, twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
 };
 } // namespace hgpm
 #endif

@@ -98,15 +98,15 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * DEP_TILE;
     const PosSplit r0 = pos[p0];
+    // pass 1: load this thread's particles (kept in registers) and the bounding box of
+    // the tile's cells relative to r0
     int rx[DEP_PPT], ry[DEP_PPT], rz[DEP_PPT];
     float fx[DEP_PPT], fy[DEP_PPT], fz[DEP_PPT];
-    bool ok[DEP_PPT];
     int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
         const int64_t j = p0 + tid + k * DEP_NT;
-        ok[k] = j < n;
-        PosSplit p = ok[k] ? pos[j] : r0;
+        const PosSplit p = j < n ? pos[j] : r0;
         rx[k] = wrap_rel(p.cx - r0.cx, ng);
         ry[k] = wrap_rel(p.cy - r0.cy, ng);
         rz[k] = wrap_rel(p.cz - r0.cz, ng);
@@ -135,6 +135,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
     const int ty = min(e[1], WMAX / e[2]);
     const int tx = min(e[0], WMAX / (ty * e[2]));
     long long local = 0;
+    // pass 2 (per slab): add the corners of each particle in LDS
     for (int sx0 = 0; sx0 < e[0]; sx0 += tx) {
         const int nx = min(tx, e[0] - sx0);
         for (int sy0 = 0; sy0 < e[1]; sy0 += ty) {
@@ -144,11 +145,15 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < DEP_PPT; k++) {
-                if (!ok[k]) continue;
+                // the weights are slab-invariant; an opaque copy of the offsets stops the
+                // compiler from hoisting all 64 of them out of the slab loop (register spill)
+                float ox = fx[k], oy = fy[k], oz = fz[k];
+                asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+                if (p0 + tid + k * DEP_NT >= n) continue;
                 const int bx = rx[k] - lo[0] - sx0, by = ry[k] - lo[1] - sy0, bz = rz[k] - lo[2];
-                const float gx[2] = {1.0f - fx[k], fx[k]};
-                const float gy[2] = {1.0f - fy[k], fy[k]};
-                const float gz[2] = {1.0f - fz[k], fz[k]};
+                const float gx[2] = {1.0f - ox, ox};
+                const float gy[2] = {1.0f - oy, oy};
+                const float gz[2] = {1.0f - oz, oz};
 #pragma unroll
                 for (int x = 0; x < 2; x++) {
                     const int lx = bx + x;
@@ -157,25 +162,40 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
                     for (int y = 0; y < 2; y++) {
                         const int ly = by + y;
                         if (ly < 0 || ly >= ny) continue;
-#pragma unroll
-                        for (int z = 0; z < 2; z++) {
-                            const long long q = fixed_weight(gx[x], gy[y], gz[z], mass);
-                            local += q;
-                            atomicAdd(&win[(lx * ny + ly) * e[2] + bz + z], (unsigned long long)q);
-                        }
+                        unsigned long long* row = &win[(lx * ny + ly) * e[2] + bz];
+                        const long long q0 = fixed_weight(gx[x], gy[y], gz[0], mass);
+                        const long long q1 = fixed_weight(gx[x], gy[y], gz[1], mass);
+                        local += q0 + q1;
+                        atomicAdd(row, (unsigned long long)q0);
+                        atomicAdd(row + 1, (unsigned long long)q1);
                     }
                 }
             }
             __syncthreads();
-            const int nyz = ny * e[2];
-            for (int i = tid; i < vol; i += DEP_NT) {
-                const unsigned long long v = win[i];
-                if (!v) continue;
-                const int ix = i / nyz, rem = i - ix * nyz, iy = rem / e[2], iz = rem - iy * e[2];
-                const int gx = wrap_cell(r0.cx + lo[0] + sx0 + ix, ng);
-                const int gy = wrap_cell(r0.cy + lo[1] + sy0 + iy, ng);
-                const int gz = wrap_cell(r0.cz + lo[2] + iz, ng);
-                atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
+            // flush: thread walks cells tid, tid + DEP_NT, ... of the slab window with
+            // incremental (ix, iy, iz) so no division runs inside the loop
+            {
+                const int ez = e[2];
+                const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
+                int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
+                const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+                for (int i = tid; i < vol; i += DEP_NT) {
+                    const unsigned long long v = win[i];
+                    if (v) {
+                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+                        atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
+                    }
+                    iz += sz;
+                    int cy = 0;
+                    if (iz >= ez) { iz -= ez; cy = 1; }
+                    iy += sy + cy;
+                    int cx = 0;
+                    if (iy >= ny) { iy -= ny; cx = 1; }
+                    ix += sx + cx;
+                }
             }
             __syncthreads();
         }
@@ -185,47 +205,141 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
 }
 
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
-// summed over the 8 corners in (x,y,z) order, then optionally the drift.
-__global__ __launch_bounds__(BS) void k_gather_kick(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
-                                                    int ng, const float2* __restrict fxy,
-                                                    const float* __restrict fz, double deltaT, double fscal,
-                                                    int do_drift, float pf) {
-    int64_t i = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if (i >= n) return;
-    PosSplit p = pos[i];
-    const int x1 = p.cx + 1 == ng ? 0 : p.cx + 1;
-    const int y1 = p.cy + 1 == ng ? 0 : p.cy + 1;
-    const int z1 = p.cz + 1 == ng ? 0 : p.cz + 1;
-    const float gx[2] = {1.0f - p.fx, p.fx};
-    const float gy[2] = {1.0f - p.fy, p.fy};
-    const float gz[2] = {1.0f - p.fz, p.fz};
-    const int xs[2] = {p.cx, x1}, ys[2] = {p.cy, y1}, zs[2] = {p.cz, z1};
-    float2 f2[8];
-    float f1[8];
+// summed over the 8 corners in (x,y,z) order, then optionally the drift.  A workgroup
+// takes one Lagrangian tile: the force cells covering the tile's bounding box are
+// staged in LDS with coalesced loads and the corners are read from there; a tile
+// whose box exceeds GMAX cells reads its corners from global memory instead.  Both
+// paths do the same FP32 arithmetic in the same order.
+constexpr int GT_NT = 1024;
+constexpr int GT_PPT = 4;
+constexpr int GT_TILE = GT_NT * GT_PPT;
+constexpr int GMAX = 6656; // 78 KB of (float2, float) per workgroup
+static_assert(GT_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
+
+__global__ __launch_bounds__(GT_NT) void k_gather_tiled(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
+                                                        int ng, const float2* __restrict fxy,
+                                                        const float* __restrict fz, double deltaT, double fscal,
+                                                        int do_drift, float pf) {
+    __shared__ float2 sxy[GMAX];
+    __shared__ float sz[GMAX];
+    __shared__ int red[6][GT_NT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
+    const PosSplit r0 = pos[p0];
+    PosSplit P[GT_PPT];
+    float V[GT_PPT][3];
+    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        int64_t id = gidx(ng, xs[c >> 2], ys[(c >> 1) & 1], zs[c & 1]);
-        f2[c] = fxy[id];
-        f1[c] = fz[id];
+    for (int k = 0; k < GT_PPT; k++) {
+        const int64_t j = p0 + tid + k * GT_NT;
+        P[k] = j < n ? pos[j] : r0;
+        V[k][0] = j < n ? vel[3 * j] : 0.0f;
+        V[k][1] = j < n ? vel[3 * j + 1] : 0.0f;
+        V[k][2] = j < n ? vel[3 * j + 2] : 0.0f;
+        const int ax = wrap_rel(P[k].cx - r0.cx, ng), ay = wrap_rel(P[k].cy - r0.cy, ng),
+                  az = wrap_rel(P[k].cz - r0.cz, ng);
+        mn[0] = min(mn[0], ax); mx[0] = max(mx[0], ax);
+        mn[1] = min(mn[1], ay); mx[1] = max(mx[1], ay);
+        mn[2] = min(mn[2], az); mx[2] = max(mx[2], az);
     }
-    float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        float w = gx[c >> 2] * gy[(c >> 1) & 1];
-        w = w * gz[c & 1];
-        float mul = (float)(((double)w * deltaT) * fscal);
-        float t;
-        t = mul * f2[c].x; dvx = dvx + t;
-        t = mul * f2[c].y; dvy = dvy + t;
-        t = mul * f1[c]; dvz = dvz + t;
+    for (int d = 0; d < 3; d++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[d] = min(mn[d], __shfl_xor(mn[d], off, 64));
+            mx[d] = max(mx[d], __shfl_xor(mx[d], off, 64));
+        }
+        if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
     }
-    float vx = vel[3 * i] + dvx, vy = vel[3 * i + 1] + dvy, vz = vel[3 * i + 2] + dvz;
-    vel[3 * i] = vx;
-    vel[3 * i + 1] = vy;
-    vel[3 * i + 2] = vz;
-    if (do_drift) {
-        drift_one(p, vx, vy, vz, pf, ng);
-        pos[i] = p;
+    __syncthreads();
+    int lo[3], e[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int a = red[d][0], b = red[3 + d][0];
+        for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); b = max(b, red[3 + d][w]); }
+        lo[d] = a;
+        e[d] = b - a + 2;
+    }
+    const int vol = e[0] * e[1] * e[2];
+    const bool staged = vol <= GMAX;
+    if (staged) {
+        const int ez = e[2], ny = e[1];
+        const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % ny, sx = syx / ny;
+        int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
+        const int bx0 = r0.cx + lo[0], by0 = r0.cy + lo[1], bz0 = r0.cz + lo[2];
+        for (int i = tid; i < vol; i += GT_NT) {
+            int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+            gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+            const int64_t g = gidx(ng, gx, gy, gz);
+            sxy[i] = fxy[g];
+            sz[i] = fz[g];
+            iz += sz_;
+            int cy = 0;
+            if (iz >= ez) { iz -= ez; cy = 1; }
+            iy += sy + cy;
+            int cx = 0;
+            if (iy >= ny) { iy -= ny; cx = 1; }
+            ix += sx + cx;
+        }
+        __syncthreads();
+    }
+    const int64_t ng2 = (int64_t)ng * ng;
+#pragma unroll
+    for (int k = 0; k < GT_PPT; k++) {
+        // one particle at a time: interleaving all eight needs more registers than
+        // two resident workgroups per CU allow
+        __builtin_amdgcn_sched_barrier(0);
+        const int64_t j = p0 + tid + k * GT_NT;
+        if (j >= n) continue;
+        PosSplit p = P[k];
+        const float gx[2] = {1.0f - p.fx, p.fx};
+        const float gy[2] = {1.0f - p.fy, p.fy};
+        const float gz[2] = {1.0f - p.fz, p.fz};
+        float2 f2[8];
+        float f1[8];
+        if (staged) {
+            const int lx = wrap_rel(p.cx - r0.cx, ng) - lo[0];
+            const int ly = wrap_rel(p.cy - r0.cy, ng) - lo[1];
+            const int lz = wrap_rel(p.cz - r0.cz, ng) - lo[2];
+            const int b0 = (lx * e[1] + ly) * e[2] + lz;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int id = b0 + ((c >> 2) ? e[1] * e[2] : 0) + (((c >> 1) & 1) ? e[2] : 0) + (c & 1);
+                f2[c] = sxy[id];
+                f1[c] = sz[id];
+            }
+        } else {
+            const int64_t base = (int64_t)p.cx * ng2 + (int64_t)p.cy * ng + p.cz;
+            const int64_t dx = p.cx + 1 == ng ? -(int64_t)(ng - 1) * ng2 : ng2;
+            const int64_t dy = p.cy + 1 == ng ? -(int64_t)(ng - 1) * ng : ng;
+            const int64_t dz = p.cz + 1 == ng ? -(int64_t)(ng - 1) : 1;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int64_t id = base + ((c >> 2) ? dx : 0) + (((c >> 1) & 1) ? dy : 0) + ((c & 1) ? dz : 0);
+                f2[c] = fxy[id];
+                f1[c] = fz[id];
+            }
+        }
+        float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            float w = gx[c >> 2] * gy[(c >> 1) & 1];
+            w = w * gz[c & 1];
+            const float mul = (float)(((double)w * deltaT) * fscal);
+            float t;
+            t = mul * f2[c].x; dvx = dvx + t;
+            t = mul * f2[c].y; dvy = dvy + t;
+            t = mul * f1[c]; dvz = dvz + t;
+        }
+        const float vx = V[k][0] + dvx, vy = V[k][1] + dvy, vz = V[k][2] + dvz;
+        vel[3 * j] = vx;
+        vel[3 * j + 1] = vy;
+        vel[3 * j + 2] = vz;
+        if (do_drift) {
+            drift_one(p, vx, vy, vz, pf, ng);
+            pos[j] = p;
+        }
     }
 }
 
@@ -345,8 +459,8 @@ void launch_deposit(const PosSplitH* pos, int64_t n, int ng, float mass, long lo
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, hipStream_t s) {
     if (!n) return;
-    k_gather_kick<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift,
-                                                    pf);
+    unsigned nb = (unsigned)((n + GT_TILE - 1) / GT_TILE);
+    k_gather_tiled<<<nb, GT_NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf);
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {

@@ -230,9 +230,17 @@ int hgpm_set_timing(hgpm_ctx* ctx, int enabled);
 int hgpm_get_timings(hgpm_ctx* ctx, char* names, int names_len, double* total_ms,
                      int64_t* launches, int max, int* n);
 int hgpm_reset_timings(hgpm_ctx* ctx);
-/* Tunables (autotuner): key = "fft_cols", "deposit_chunk"...; returns ARG for unknown */
+/* Tunables: "fft_cols" and "fft_xsolve" select the FFT column-pass and fused
+   x-solve shapes (columns per tile x threads: 0 = 4x256, 1 = 4x512, 2 = 8x512,
+   3 = 16x512 where the tile fits LDS); "fft_variants" (get only) is the count valid for
+   this grid.  Unknown keys return HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
+/* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each
+   FFT variant on this context's grids (reps launches each), keeps the fastest, and
+   reports per-variant ms in ms_cols / ms_xsolve (4 entries each, may be NULL).
+   Clobbers the density and force grids. */
+int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,33 @@ def test_solve_fp32_vs_fp64_oracle(hg, orc, ng):
     assert err < 2e-5, err
 
 
+@pytest.mark.parametrize("ng", [32, 96, 128])
+def test_fft_variants_agree(hg, orc, ng):
+    """Every column-pass / x-solve variant the autotuner may pick gives the same physics."""
+    case(f"fft variants ng={ng}")
+    rng = np.random.default_rng(ng + 21)
+    with mk(hg, ng) as m:
+        nv = m.get_tunable("fft_variants")
+        assert nv >= 3
+        cell, off = rand_split(rng, m.N, ng, clustered=True)
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        q = m.get_density()
+        Fo = orc.solve_gradient(ng, q * 2.0 ** -24)
+        x = (rng.standard_normal((ng,) * 3) + 1j * rng.standard_normal((ng,) * 3)).astype(np.complex64)
+        for v in range(nv):
+            m.set_tunable("fft_cols", v)
+            m.set_tunable("fft_xsolve", v)
+            m.set_density(q)
+            m.solve_gradient()
+            F = m.get_force().astype(np.float64)
+            err = np.sqrt(((F - Fo) ** 2).mean()) / np.sqrt((Fo ** 2).mean())
+            assert err < 2e-5, (v, err)
+            assert rel_l2(m.fft(x, -1), np.fft.fftn(x.astype(np.complex128))) < 2e-6, v
+        res = m.autotune(2)
+        assert 0 <= res["fft_cols"] < nv and 0 <= res["fft_xsolve"] < nv
+
+
 @pytest.mark.parametrize("ng", [32, 64])
 def test_solve_fp64_path(hg, orc, ng):
     case(f"solve fp64 ng={ng}")
