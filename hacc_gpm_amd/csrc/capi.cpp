@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -44,7 +45,16 @@ struct hgpm_ctx {
     float2* A = nullptr;             // spectrum / Fz^ (8 B per cell)
     float2* B = nullptr;             // int64 density / Fx^+iFy^ / (Fx,Fy) (8 B per cell)
     float* fz = nullptr;             // Fz (4 B per cell)
-    unsigned long long* total = nullptr;
+    unsigned long long* ctl = nullptr;   // deposit control words; ctl[0] = deposited total
+    unsigned long long* total = nullptr; // = ctl
+    // deposit arena route (k_particles.hip): window arena, per-owner-block chunk slots
+    unsigned long long* arena = nullptr;
+    unsigned long long arena_alloc = 0, arena_cap = 0; // cells allocated / usable
+    void* dslots = nullptr;
+    unsigned* dcount = nullptr;
+    unsigned* deferred = nullptr;
+    unsigned long long* tile_total = nullptr;
+    int nslots = HGPM_DEP_SLOTS;
     float2 *twf = nullptr, *twr = nullptr, *twf_ic = nullptr;
     float *sk = nullptr, *hs = nullptr;
     // FP64 plain path
@@ -203,14 +213,37 @@ float ref_mass(const hgpm_ctx* c) {
 }
 
 // ------------------------------------------------------------ FP32 ops ----
+DepositBuffers dep_buffers(hgpm_ctx* c) {
+    DepositBuffers b;
+    b.arena = c->arena;
+    b.cap = c->arena_cap;
+    b.slots = c->dslots;
+    b.count = c->dcount;
+    b.nslots = c->nslots;
+    b.deferred = c->deferred;
+    b.ctl = c->ctl;
+    b.tile_total = c->tile_total;
+    return b;
+}
+
+// Density of the current positions into B (int64, every cell written) and its total
+// into ctl[0]: windows -> owner blocks -> deferred tiles.
 int f32_deposit(hgpm_ctx* c) {
-    {
-        Timed t(c, "zero_density");
-        launch_zero_u64((unsigned long long*)c->B, (size_t)c->G, c->st);
-        launch_zero_u64(c->total, 1, c->st);
-    }
+    const DepositBuffers b = dep_buffers(c);
     Timed t(c, "deposit");
-    launch_deposit(c->pos, c->N, c->ng, ref_mass(c), (long long*)c->B, c->total, c->st);
+    launch_zero_u64(c->ctl, HGPM_DEP_CTL, c->st);
+    {
+        Timed t1(c, "deposit_windows");
+        launch_deposit_windows(c->pos, c->N, c->ng, ref_mass(c), b, c->st);
+    }
+    {
+        Timed t2(c, "deposit_owner");
+        launch_deposit_owner(c->ng, (long long*)c->B, b, c->st);
+    }
+    {
+        Timed t3(c, "deposit_deferred");
+        launch_deposit_deferred(c->pos, c->N, c->ng, ref_mass(c), (long long*)c->B, b, c->st);
+    }
     LAUNCHCHK(c);
     return HGPM_OK;
 }
@@ -378,8 +411,28 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
             (rc = dalloc(c, &c->A, sizeof(float2) * G, "grid A")) ||
             (rc = dalloc(c, &c->B, sizeof(float2) * G, "grid B")) ||
             (rc = dalloc(c, &c->fz, sizeof(float) * G, "Fz")) ||
-            (rc = dalloc(c, &c->total, sizeof(unsigned long long), "total")))
+            (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")))
             return bail(rc);
+        c->total = c->ctl;
+        {
+            // window arena: 4 cells per grid cell holds every tile window of the runs
+            // measured here through z = 0 (the windows' total grows from ~1.4 G to ~4 G
+            // cells as structure forms); tiles beyond it take the atomic route
+            const size_t nblk = (size_t)(p->ng / 16) * (p->ng / 16) * (p->ng / 16);
+            const size_t ntiles = N / (16 * 16 * 16);
+            c->arena_alloc = 4ull * G;
+            c->arena_cap = c->arena_alloc;
+            if ((rc = dalloc(c, &c->arena, sizeof(unsigned long long) * c->arena_alloc, "deposit arena")) ||
+                (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
+                (rc = dalloc(c, &c->dcount, sizeof(unsigned) * nblk, "deposit counts")) ||
+                (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||
+                (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")))
+                return bail(rc);
+            if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess) {
+                c->err = "hipMemset failed";
+                return bail(HGPM_ERR_HIP);
+            }
+        }
         if ((rc = upload_twiddles<float2>(c, c->fft, &c->twf, &c->twr))) return bail(rc);
         if (c->np != c->ng) {
             if ((rc = upload_twiddles<float2>(c, c->fftic, &c->twf_ic, nullptr))) return bail(rc);
@@ -440,7 +493,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->total, c->twf, c->twr, c->sk, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->twf, c->twr, c->sk, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -1003,6 +1056,17 @@ int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
         (!strcmp(key, "fft_cols") ? c->var_cols : c->var_xs) = value;
         return HGPM_OK;
     }
+    if (!strcmp(key, "deposit_arena")) { // usable arena cells; < 0 restores the allocation
+        if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "deposit_arena needs an FP32 context");
+        c->arena_cap = value < 0 ? c->arena_alloc : std::min<unsigned long long>((unsigned long long)value, c->arena_alloc);
+        return HGPM_OK;
+    }
+    if (!strcmp(key, "deposit_slots")) {
+        if (value < 1 || value > HGPM_DEP_SLOTS)
+            return fail(c, HGPM_ERR_ARG, "deposit_slots must be in [1," + std::to_string(HGPM_DEP_SLOTS) + "]");
+        c->nslots = value;
+        return HGPM_OK;
+    }
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
 }
 int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
@@ -1010,6 +1074,19 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     if (!strcmp(key, "fft_cols")) { *value = c->var_cols; return HGPM_OK; }
     if (!strcmp(key, "fft_xsolve")) { *value = c->var_xs; return HGPM_OK; }
     if (!strcmp(key, "fft_variants")) { *value = c->fft ? c->fft->valid_variants : 0; return HGPM_OK; }
+    if (!strcmp(key, "deposit_arena")) {
+        *value = (int)std::min<unsigned long long>(c->arena_cap, 0x7fffffffull);
+        return HGPM_OK;
+    }
+    if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
+    if (!strcmp(key, "deposit_deferred")) { // tiles that took the atomic route in the last deposit
+        if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
+        unsigned long long v[2];
+        HIPCHK(c, hipMemcpyAsync(v, c->ctl, sizeof(v), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        *value = (int)v[1];
+        return HGPM_OK;
+    }
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
 }
 

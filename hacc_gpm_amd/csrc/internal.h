@@ -43,8 +43,32 @@ struct PosSplitH { int cx, cy, cz; float fx, fy, fz; };
 void launch_zero_i64(long long* p, size_t n, hipStream_t s);
 void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 void launch_drift(PosSplitH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
-void launch_deposit(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
-                    unsigned long long* total, hipStream_t s);
+// Deposit buffers.  np3 is a multiple of HGPM_TL^3 and ng of 16 (supported sizes).
+//   arena    : cap int64 cells for the tiles' windows;
+//   slots    : HGPM_DEP_SLOTS chunk descriptors (16 B) per 16^3 owner block;
+//   count    : chunks listed per owner block, zero between deposits;
+//   deferred : one entry per tile;
+//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] deposited total,
+//              [1] deferred tiles, [4 + s] cells reserved in sub-arena s;
+//   tile_total: deposited mass per tile.
+#define HGPM_DEP_SLOTS 64
+#define HGPM_DEP_NSUB 64
+#define HGPM_DEP_CTL (4 + HGPM_DEP_NSUB)
+struct DepositBuffers {
+    unsigned long long* arena;
+    unsigned long long cap;
+    void* slots;
+    unsigned* count;
+    int nslots;
+    unsigned* deferred;
+    unsigned long long* ctl;
+    unsigned long long* tile_total;
+};
+void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
+                            hipStream_t s);
+void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s);
+void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
+                             const DepositBuffers& b, hipStream_t s);
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order

@@ -65,19 +65,36 @@ __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, 
     return (long long)__float2ll_rn(w * 16777216.0f);
 }
 
-// Tiled deposit.  A workgroup takes one Lagrangian tile (DEP_TILE consecutive particles
-// of the internal order, a compact block in space), finds the bounding box of their
-// cells relative to the first particle's cell (periodic-unwrapped), accumulates the 8
-// fixed-point CIC weights of every particle into an int64 window in LDS (ds_add_u64)
-// and adds the non-empty window cells to the global density with coalesced int64
-// atomics.  A tile whose window exceeds WMAX cells is processed in slabs (x, then y):
-// each pass accumulates the corners that fall inside the slab.  Integer sums are
-// order-free, so the result is bit-identical to any sequential order.
+// ----------------------------------------------------------------------------------
+// Deposit.  A workgroup takes one Lagrangian tile (DEP_TILE consecutive particles of the
+// internal order, a compact block in space), finds the bounding box of the cells its
+// particles touch, relative to the first particle's cell (periodic-unwrapped), and
+// accumulates the 8 fixed-point CIC weights of every particle into an int64 window over
+// that box in LDS (ds_add_u64).  A box larger than WMAX cells is processed in slabs
+// (x, then y); each slab pass accumulates the corners that fall inside the slab.
+//
+// The finished window cells reach the density grid without global atomics:
+//   k_deposit_windows : stores each tile's window to an arena with plain stores and cuts
+//                       it along the OB^3 owner blocks of the grid; each piece (chunk)
+//                       gets a descriptor in its owner block's slot list;
+//   k_deposit_owner   : one workgroup per owner block adds the chunks listed for it in
+//                       LDS and writes the block's cells once (the grid is not cleared
+//                       beforehand: every cell is written);
+//   k_deposit_deferred: tiles that did not fit (arena full, slot list full, more than
+//                       DEP_MAXCH chunks) are deposited afterwards with int64 atomics.
+// Integer sums are order-free, so every route gives the bits of a sequential sum.
 constexpr int DEP_NT = 512;
 constexpr int DEP_PPT = 8;
 constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
 constexpr int WMAX = 9216;                 // 72 KB of int64 per workgroup
+constexpr int OB = 16;                     // owner block edge (cells)
+constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
+constexpr int OWN_NT = 256;
+constexpr int OWN_U = 8;                // arena loads in flight per lane
+constexpr int DEP_NSUB = HGPM_DEP_NSUB; // sub-arenas
+constexpr int DEP_CTL_SUB = 4;          // first sub-arena counter in ctl
 static_assert(DEP_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
+static_assert(HGPM_DEP_SLOTS <= 0xFFFF, "slot index");
 
 __device__ __forceinline__ int wrap_rel(int d, int ng) {
     const int h = ng >> 1;
@@ -90,30 +107,31 @@ __device__ __forceinline__ int wrap_cell(int v, int ng) {
     return v < 0 ? v + ng : v;
 }
 
-__global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __restrict pos, int64_t n, int ng,
-                                                          float mass, unsigned long long* __restrict rho,
-                                                          unsigned long long* __restrict total) {
-    __shared__ unsigned long long win[WMAX];
-    __shared__ int red[6][DEP_NT / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * DEP_TILE;
-    const PosSplit r0 = pos[p0];
-    // pass 1: load this thread's particles (kept in registers) and the bounding box of
-    // the tile's cells relative to r0
+// Registers of one tile's particles: cell relative to the first particle's cell r0
+// (periodic-unwrapped) and offsets.
+struct DepRegs {
     int rx[DEP_PPT], ry[DEP_PPT], rz[DEP_PPT];
     float fx[DEP_PPT], fy[DEP_PPT], fz[DEP_PPT];
+};
+
+// Loads the tile starting at p0 and reduces the box of its cells: lo = minimum
+// relative cell, e = extent + 1 (the upper CIC neighbour).
+__device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int64_t p0, int64_t n, int ng,
+                                             const PosSplit& r0, DepRegs& R, int (*red)[DEP_NT / 64], int lo[3],
+                                             int e[3]) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
         const int64_t j = p0 + tid + k * DEP_NT;
         const PosSplit p = j < n ? pos[j] : r0;
-        rx[k] = wrap_rel(p.cx - r0.cx, ng);
-        ry[k] = wrap_rel(p.cy - r0.cy, ng);
-        rz[k] = wrap_rel(p.cz - r0.cz, ng);
-        fx[k] = p.fx; fy[k] = p.fy; fz[k] = p.fz;
-        mn[0] = min(mn[0], rx[k]); mx[0] = max(mx[0], rx[k]);
-        mn[1] = min(mn[1], ry[k]); mx[1] = max(mx[1], ry[k]);
-        mn[2] = min(mn[2], rz[k]); mx[2] = max(mx[2], rz[k]);
+        R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
+        R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
+        R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
+        R.fx[k] = p.fx; R.fy[k] = p.fy; R.fz[k] = p.fz;
+        mn[0] = min(mn[0], R.rx[k]); mx[0] = max(mx[0], R.rx[k]);
+        mn[1] = min(mn[1], R.ry[k]); mx[1] = max(mx[1], R.ry[k]);
+        mn[2] = min(mn[2], R.rz[k]); mx[2] = max(mx[2], R.rz[k]);
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -124,18 +142,28 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
         if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
     }
     __syncthreads();
-    int lo[3], e[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         int a = red[d][0], b = red[3 + d][0];
         for (int w = 1; w < DEP_NT / 64; w++) { a = min(a, red[d][w]); b = max(b, red[3 + d][w]); }
         lo[d] = a;
-        e[d] = b - a + 2; // + the upper CIC neighbour
+        e[d] = b - a + 2;
     }
+}
+
+// Accumulates the tile's CIC weights slab by slab in `win` and hands each finished
+// slab to the flush: ARENA stores every cell of the slab to arena[base + window index]
+// (window order x, y, z with extents e), otherwise the non-zero cells are added to the
+// grid with int64 atomics.  Returns this thread's share of the deposited mass.
+template <bool ARENA>
+__device__ __forceinline__ long long dep_window(unsigned long long* win, const DepRegs& R, const int lo[3],
+                                                const int e[3], const PosSplit& r0, int64_t p0, int64_t n, int ng,
+                                                float mass, unsigned long long* __restrict dst,
+                                                unsigned long long base) {
+    const int tid = threadIdx.x;
     const int ty = min(e[1], WMAX / e[2]);
     const int tx = min(e[0], WMAX / (ty * e[2]));
     long long local = 0;
-    // pass 2 (per slab): add the corners of each particle in LDS
     for (int sx0 = 0; sx0 < e[0]; sx0 += tx) {
         const int nx = min(tx, e[0] - sx0);
         for (int sy0 = 0; sy0 < e[1]; sy0 += ty) {
@@ -147,10 +175,10 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
             for (int k = 0; k < DEP_PPT; k++) {
                 // the weights are slab-invariant; an opaque copy of the offsets stops the
                 // compiler from hoisting all 64 of them out of the slab loop (register spill)
-                float ox = fx[k], oy = fy[k], oz = fz[k];
+                float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
                 asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
                 if (p0 + tid + k * DEP_NT >= n) continue;
-                const int bx = rx[k] - lo[0] - sx0, by = ry[k] - lo[1] - sy0, bz = rz[k] - lo[2];
+                const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0, bz = R.rz[k] - lo[2];
                 const float gx[2] = {1.0f - ox, ox};
                 const float gy[2] = {1.0f - oy, oy};
                 const float gz[2] = {1.0f - oz, oz};
@@ -172,8 +200,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
                 }
             }
             __syncthreads();
-            // flush: thread walks cells tid, tid + DEP_NT, ... of the slab window with
-            // incremental (ix, iy, iz) so no division runs inside the loop
+            // flush: thread walks cells tid, tid + DEP_NT, ... of the slab with incremental
+            // (ix, iy, iz) so no division runs inside the loop
             {
                 const int ez = e[2];
                 const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
@@ -181,12 +209,14 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
                 const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
                 for (int i = tid; i < vol; i += DEP_NT) {
                     const unsigned long long v = win[i];
-                    if (v) {
+                    if (ARENA) {
+                        dst[base + ((uint64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz] = v;
+                    } else if (v) {
                         int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
                         gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
                         gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
                         gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                        atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
+                        atomicAdd(&dst[gidx(ng, gx, gy, gz)], v);
                     }
                     iz += sz;
                     int cy = 0;
@@ -200,26 +230,243 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_tiled(const PosSplit* __rest
             __syncthreads();
         }
     }
+    return local;
+}
+
+// Workgroup sum of the threads' deposited mass, stored by thread 0 as the tile's total
+// (a plain store: one word per tile, summed by k_deposit_total).
+__device__ __forceinline__ void dep_tile_total(long long local, long long* s_tot, unsigned long long* out) {
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
-    if (lane == 0 && local != 0) atomicAdd(total, (unsigned long long)local);
+    if ((threadIdx.x & 63) == 0) s_tot[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int w = 0; w < DEP_NT / 64; w++) t += s_tot[w];
+        *out = (unsigned long long)t;
+    }
+}
+
+// Chunk c (0 <= c < nc0*nc1*nc2) of a window with unwrapped origin A and extents e:
+// the owner block it lies in and its sub-box (origin inside the block, extents, origin
+// inside the window).
+struct Chunk { int blk; int o[3], m[3], w[3]; };
+__device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3], const int c0[3], const int nc[3],
+                                           int nb) {
+    Chunk ch;
+    const int ci[3] = {c / (nc[1] * nc[2]), (c / nc[2]) % nc[1], c % nc[2]};
+    int b[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int C = c0[d] + ci[d];
+        const int a = max(A[d], C * OB), z = min(A[d] + e[d], C * OB + OB);
+        ch.o[d] = a - C * OB;
+        ch.m[d] = z - a;
+        ch.w[d] = a - A[d];
+        b[d] = wrap_cell(C, nb);
+    }
+    ch.blk = (b[0] * nb + b[1]) * nb + b[2];
+    return ch;
+}
+
+// ctl[1] = deferred tiles, ctl[DEP_CTL_SUB + s] = cells reserved in sub-arena s.  The
+// arena is split into DEP_NSUB equal sub-arenas (tile % DEP_NSUB picks one) so that no
+// single counter takes every tile's reservation.
+__global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __restrict pos, int64_t n, int ng,
+                                                            float mass, unsigned long long* __restrict arena,
+                                                            unsigned long long cap, uint4* __restrict slots,
+                                                            unsigned* __restrict count, int nslots,
+                                                            unsigned* __restrict deferred,
+                                                            unsigned long long* __restrict ctl,
+                                                            unsigned long long* __restrict tile_total) {
+    __shared__ unsigned long long win[WMAX];
+    __shared__ int red[6][DEP_NT / 64];
+    __shared__ long long s_tot[DEP_NT / 64];
+    __shared__ unsigned long long s_base;
+    __shared__ int s_state; // 0 arena route, 1 deferred (nothing reserved), 2 deferred (slots voided)
+    __shared__ unsigned short s_slot[DEP_MAXCH];
+    const int tid = threadIdx.x;
+    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t p0 = (int64_t)tile * DEP_TILE;
+    const PosSplit r0 = pos[p0];
+    DepRegs R;
+    int lo[3], e[3];
+    dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
+    const int nb = ng / OB;
+    const int A[3] = {r0.cx + lo[0], r0.cy + lo[1], r0.cz + lo[2]};
+    int c0[3], nc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        c0[d] = A[d] >> 4; // floor division by OB (arithmetic shift)
+        nc[d] = ((A[d] + e[d] - 1) >> 4) - c0[d] + 1;
+    }
+    const int nch = nc[0] * nc[1] * nc[2];
+    const unsigned long long V = (unsigned long long)e[0] * e[1] * e[2];
+    if (tid == 0) {
+        int st = 1;
+        if (nch <= DEP_MAXCH) {
+            const unsigned long long sub_cap = cap / DEP_NSUB;
+            const int sub = tile % DEP_NSUB;
+            const unsigned long long b = atomicAdd(&ctl[DEP_CTL_SUB + sub], V);
+            s_base = (unsigned long long)sub * sub_cap + b;
+            st = (b + V <= sub_cap) ? 0 : 1;
+        }
+        s_state = st;
+    }
+    __syncthreads();
+    if (s_state == 0) {
+        for (int c = tid; c < nch; c += DEP_NT) {
+            const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
+            const unsigned s = atomicAdd(&count[ch.blk], 1u);
+            s_slot[c] = (unsigned short)(s < (unsigned)nslots ? s : 0xFFFFu);
+            if (s >= (unsigned)nslots) s_state = 2;
+        }
+    }
+    __syncthreads();
+    const int state = s_state;
+    if (state != 1) {
+        const unsigned long long base = s_base;
+        for (int c = tid; c < nch; c += DEP_NT) {
+            if (s_slot[c] == 0xFFFFu) continue;
+            const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
+            const unsigned long long off = base + ((uint64_t)ch.w[0] * e[1] + ch.w[1]) * e[2] + ch.w[2];
+            uint4 de;
+            de.x = (unsigned)off;
+            de.y = (unsigned)(off >> 32);
+            de.z = (unsigned)(ch.o[0] | ch.o[1] << 4 | ch.o[2] << 8 | (ch.m[0] - 1) << 12 | (ch.m[1] - 1) << 16 |
+                              (ch.m[2] - 1) << 20) |
+                   (state == 2 ? 0x80000000u : 0u);
+            de.w = (unsigned)e[1] | (unsigned)e[2] << 16;
+            slots[(size_t)ch.blk * nslots + s_slot[c]] = de;
+        }
+    }
+    if (state != 0) {
+        if (tid == 0) {
+            deferred[atomicAdd(&ctl[1], 1ull)] = (unsigned)tile;
+            tile_total[tile] = 0ull; // written by k_deposit_deferred
+        }
+        return;
+    }
+    const long long local = dep_window<true>(win, R, lo, e, r0, p0, n, ng, mass, arena, s_base);
+    dep_tile_total(local, s_tot, &tile_total[tile]);
+}
+
+// One workgroup per OB^3 owner block: sum the listed chunks in LDS, write every cell.
+// Resets the block's chunk count for the next deposit.
+__global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long long* __restrict arena,
+                                                          const uint4* __restrict slots,
+                                                          unsigned* __restrict count, int nslots, int ng,
+                                                          unsigned long long* __restrict rho) {
+    __shared__ unsigned long long acc[OB * OB * OB];
+    __shared__ uint4 desc[HGPM_DEP_SLOTS];
+    __shared__ int s_n;
+    const int nb = ng / OB;
+    const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid == 0) {
+        const unsigned c = count[blk];
+        count[blk] = 0u;
+        s_n = (int)min(c, (unsigned)nslots);
+    }
+    for (int i = tid; i < OB * OB * OB; i += OWN_NT) acc[i] = 0ull;
+    __syncthreads();
+    const int nd = s_n;
+    for (int i = tid; i < nd; i += OWN_NT) desc[i] = slots[(size_t)blk * nslots + i];
+    __syncthreads();
+    for (int d = wv; d < nd; d += OWN_NT / 64) {
+        const uint4 de = desc[d];
+        if (de.z >> 31) continue;
+        const unsigned long long off = (unsigned long long)de.x | ((unsigned long long)de.y << 32);
+        const int ox = de.z & 15, oy = (de.z >> 4) & 15, oz = (de.z >> 8) & 15;
+        const int mx = ((de.z >> 12) & 15) + 1, my = ((de.z >> 16) & 15) + 1, mz = ((de.z >> 20) & 15) + 1;
+        const int e1 = (int)(de.w & 0xFFFFu), e2 = (int)(de.w >> 16);
+        const int myz = my * mz, vol = mx * myz;
+        // i -> (lx, ly, lz) by multiplication with a rounded reciprocal: exact here because
+        // i < 4096 keeps the rounding error below the 0.5/divisor margin of (i + 0.5)
+        const float rmyz = 1.0f / (float)myz, rmz = 1.0f / (float)mz;
+        for (int i0 = 0; i0 < vol; i0 += OWN_U * 64) {
+            unsigned long long v[OWN_U];
+            int at[OWN_U];
+#pragma unroll
+            for (int u = 0; u < OWN_U; u++) {
+                const int i = i0 + u * 64 + lane;
+                const int lx = (int)(((float)i + 0.5f) * rmyz);
+                const int r = i - lx * myz;
+                const int ly = (int)(((float)r + 0.5f) * rmz);
+                const int lz = r - ly * mz;
+                at[u] = i < vol ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
+                v[u] = i < vol ? arena[off + ((uint64_t)lx * e1 + ly) * e2 + lz] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < OWN_U; u++)
+                if (v[u]) atomicAdd(&acc[at[u]], v[u]);
+        }
+    }
+    __syncthreads();
+    const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+    for (int i = tid; i < OB * OB * OB; i += OWN_NT) {
+        const int lx = i >> 8, ly = (i >> 4) & 15, lz = i & 15;
+        rho[gidx(ng, bx * OB + lx, by * OB + ly, bz * OB + lz)] = acc[i];
+    }
+}
+
+// Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
+__global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __restrict pos, int64_t n, int ng,
+                                                             float mass, unsigned long long* __restrict rho,
+                                                             const unsigned long long* __restrict ctl,
+                                                             const unsigned* __restrict deferred,
+                                                             unsigned long long* __restrict tile_total) {
+    __shared__ unsigned long long win[WMAX];
+    __shared__ int red[6][DEP_NT / 64];
+    __shared__ long long s_tot[DEP_NT / 64];
+    const unsigned long long nd = ctl[1];
+    for (unsigned long long q = blockIdx.x; q < nd; q += gridDim.x) {
+        const unsigned tile = deferred[q];
+        const int64_t p0 = (int64_t)tile * DEP_TILE;
+        const PosSplit r0 = pos[p0];
+        DepRegs R;
+        int lo[3], e[3];
+        dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
+        const long long local = dep_window<false>(win, R, lo, e, r0, p0, n, ng, mass, rho, 0ull);
+        dep_tile_total(local, s_tot, &tile_total[tile]);
+        __syncthreads();
+    }
+}
+
+// ctl[0] = sum of the tiles' totals (one workgroup; integer sum, order-free).
+__global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long* __restrict tile_total,
+                                                        int64_t ntiles, unsigned long long* __restrict ctl) {
+    __shared__ unsigned long long s[1024 / 64];
+    unsigned long long t = 0;
+    for (int64_t i = threadIdx.x; i < ntiles; i += 1024) t += tile_total[i];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int w = 0; w < 1024 / 64; w++) a += s[w];
+        ctl[0] = a;
+    }
 }
 
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
 // summed over the 8 corners in (x,y,z) order, then optionally the drift.  A workgroup
-// takes one Lagrangian tile: the force cells covering the tile's bounding box are
-// staged in LDS with coalesced loads and the corners are read from there; a tile
-// whose box exceeds GMAX cells reads its corners from global memory instead.  Both
-// paths do the same FP32 arithmetic in the same order.
-constexpr int GT_NT = 1024;
+// takes GT_TILE consecutive particles (half a Lagrangian tile: an 8 x 16 x 16 block of
+// the lattice).  The force cells under the bounding box of the particles' base cells
+// (plus the upper neighbours) are staged in LDS with coalesced loads, in slabs of at
+// most GMAX cells: a slab covers a range of base planes in x (and of base rows in y)
+// plus one overlapping plane (row), so each particle finds all 8 corners in the slab
+// that holds its base cell and is processed in exactly one slab.
+constexpr int GT_NT = 512;
 constexpr int GT_PPT = 4;
 constexpr int GT_TILE = GT_NT * GT_PPT;
-constexpr int GMAX = 6656; // 78 KB of (float2, float) per workgroup
-static_assert(GT_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
+constexpr int GMAX = 6656; // 78 KB of (float2, float) per workgroup: two workgroups per CU
+constexpr int GT_STAGE = 8; // staged cells per thread per load round
+static_assert(DEP_TILE % GT_TILE == 0, "tile");
 
-__global__ __launch_bounds__(GT_NT) void k_gather_tiled(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
-                                                        int ng, const float2* __restrict fxy,
-                                                        const float* __restrict fz, double deltaT, double fscal,
-                                                        int do_drift, float pf) {
+__global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
+                                                       int ng, const float2* __restrict fxy,
+                                                       const float* __restrict fz, double deltaT, double fscal,
+                                                       int do_drift, float pf) {
     __shared__ float2 sxy[GMAX];
     __shared__ float sz[GMAX];
     __shared__ int red[6][GT_NT / 64];
@@ -251,94 +498,111 @@ __global__ __launch_bounds__(GT_NT) void k_gather_tiled(PosSplit* __restrict pos
         if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
     }
     __syncthreads();
-    int lo[3], e[3];
+    int lo[3], b[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        int a = red[d][0], b = red[3 + d][0];
-        for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); b = max(b, red[3 + d][w]); }
+        int a = red[d][0], c = red[3 + d][0];
+        for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); c = max(c, red[3 + d][w]); }
         lo[d] = a;
-        e[d] = b - a + 2;
+        b[d] = c - a + 1; // base cells per axis; the staged extent adds the upper neighbour
     }
-    const int vol = e[0] * e[1] * e[2];
-    const bool staged = vol <= GMAX;
-    if (staged) {
-        const int ez = e[2], ny = e[1];
-        const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % ny, sx = syx / ny;
-        int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
-        const int bx0 = r0.cx + lo[0], by0 = r0.cy + lo[1], bz0 = r0.cz + lo[2];
-        for (int i = tid; i < vol; i += GT_NT) {
-            int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
-            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-            gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-            const int64_t g = gidx(ng, gx, gy, gz);
-            sxy[i] = fxy[g];
-            sz[i] = fz[g];
-            iz += sz_;
-            int cy = 0;
-            if (iz >= ez) { iz -= ez; cy = 1; }
-            iy += sy + cy;
-            int cx = 0;
-            if (iy >= ny) { iy -= ny; cx = 1; }
-            ix += sx + cx;
-        }
-        __syncthreads();
-    }
-    const int64_t ng2 = (int64_t)ng * ng;
+    const int ez = b[2] + 1;
+    // slab shape: ty base rows (+1 staged) with (ty+1)*ez <= GMAX/2, then tx base planes
+    // (+1 staged) with (tx+1)*(ty+1)*ez <= GMAX; ez <= ng+1 <= 1025 keeps both >= 1
+    int ty = min(b[1], GMAX / (2 * ez) - 1);
+    ty = (b[1] + (b[1] + ty - 1) / ty - 1) / ((b[1] + ty - 1) / ty); // even split
+    int tx = min(b[0], GMAX / ((ty + 1) * ez) - 1);
+    tx = (b[0] + (b[0] + tx - 1) / tx - 1) / ((b[0] + tx - 1) / tx);
+    for (int sx0 = 0; sx0 < b[0]; sx0 += tx) {
+        const int nxs = min(tx, b[0] - sx0) + 1;
+        for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
+            const int nys = min(ty, b[1] - sy0) + 1;
+            const int vol = nxs * nys * ez;
+            {
+                // GT_STAGE cells per thread per round: all their loads are issued before
+                // the first LDS store, so a round costs one memory latency, not GT_STAGE
+                const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % nys, sx = syx / nys;
+                int iz = tid % ez, iy = (tid / ez) % nys, ix = (tid / ez) / nys;
+                const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+                for (int i0 = tid; i0 < vol; i0 += GT_STAGE * GT_NT) {
+                    unsigned g[GT_STAGE];
 #pragma unroll
-    for (int k = 0; k < GT_PPT; k++) {
-        // one particle at a time: interleaving all eight needs more registers than
-        // two resident workgroups per CU allow
-        __builtin_amdgcn_sched_barrier(0);
-        const int64_t j = p0 + tid + k * GT_NT;
-        if (j >= n) continue;
-        PosSplit p = P[k];
-        const float gx[2] = {1.0f - p.fx, p.fx};
-        const float gy[2] = {1.0f - p.fy, p.fy};
-        const float gz[2] = {1.0f - p.fz, p.fz};
-        float2 f2[8];
-        float f1[8];
-        if (staged) {
-            const int lx = wrap_rel(p.cx - r0.cx, ng) - lo[0];
-            const int ly = wrap_rel(p.cy - r0.cy, ng) - lo[1];
-            const int lz = wrap_rel(p.cz - r0.cz, ng) - lo[2];
-            const int b0 = (lx * e[1] + ly) * e[2] + lz;
+                    for (int u = 0; u < GT_STAGE; u++) {
+                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+                        g[u] = ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng + (unsigned)gz;
+                        iz += sz_;
+                        int cy = 0;
+                        if (iz >= ez) { iz -= ez; cy = 1; }
+                        iy += sy + cy;
+                        int cx = 0;
+                        if (iy >= nys) { iy -= nys; cx = 1; }
+                        ix += sx + cx;
+                    }
+                    float2 a[GT_STAGE];
+                    float c[GT_STAGE];
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int id = b0 + ((c >> 2) ? e[1] * e[2] : 0) + (((c >> 1) & 1) ? e[2] : 0) + (c & 1);
-                f2[c] = sxy[id];
-                f1[c] = sz[id];
+                    for (int u = 0; u < GT_STAGE; u++) {
+                        if (i0 + u * GT_NT < vol) {
+                            a[u] = fxy[g[u]];
+                            c[u] = fz[g[u]];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < GT_STAGE; u++) {
+                        if (i0 + u * GT_NT < vol) {
+                            sxy[i0 + u * GT_NT] = a[u];
+                            sz[i0 + u * GT_NT] = c[u];
+                        }
+                    }
+                }
             }
-        } else {
-            const int64_t base = (int64_t)p.cx * ng2 + (int64_t)p.cy * ng + p.cz;
-            const int64_t dx = p.cx + 1 == ng ? -(int64_t)(ng - 1) * ng2 : ng2;
-            const int64_t dy = p.cy + 1 == ng ? -(int64_t)(ng - 1) * ng : ng;
-            const int64_t dz = p.cz + 1 == ng ? -(int64_t)(ng - 1) : 1;
+            __syncthreads();
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                const int64_t id = base + ((c >> 2) ? dx : 0) + (((c >> 1) & 1) ? dy : 0) + ((c & 1) ? dz : 0);
-                f2[c] = fxy[id];
-                f1[c] = fz[id];
+            for (int k = 0; k < GT_PPT; k++) {
+                // one particle at a time: interleaving them needs more registers than two
+                // resident workgroups per CU allow
+                __builtin_amdgcn_sched_barrier(0);
+                const int64_t j = p0 + tid + k * GT_NT;
+                if (j >= n) continue;
+                // an opaque copy keeps the slab-invariant weights and indices of all
+                // GT_PPT particles from being hoisted out of the slab loops (registers)
+                PosSplit p = P[k];
+                asm volatile("" : "+v"(p.cx), "+v"(p.cy), "+v"(p.cz), "+v"(p.fx), "+v"(p.fy), "+v"(p.fz));
+                const int lx = wrap_rel(p.cx - r0.cx, ng) - lo[0] - sx0;
+                const int ly = wrap_rel(p.cy - r0.cy, ng) - lo[1] - sy0;
+                if (lx < 0 || lx >= nxs - 1 || ly < 0 || ly >= nys - 1) continue;
+                const int lz = wrap_rel(p.cz - r0.cz, ng) - lo[2];
+                const float gx[2] = {1.0f - p.fx, p.fx};
+                const float gy[2] = {1.0f - p.fy, p.fy};
+                const float gz[2] = {1.0f - p.fz, p.fz};
+                const int b0 = (lx * nys + ly) * ez + lz;
+                float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const int id = b0 + ((c >> 2) ? nys * ez : 0) + (((c >> 1) & 1) ? ez : 0) + (c & 1);
+                    const float2 f2 = sxy[id];
+                    const float f1 = sz[id];
+                    float w = gx[c >> 2] * gy[(c >> 1) & 1];
+                    w = w * gz[c & 1];
+                    const float mul = (float)(((double)w * deltaT) * fscal);
+                    float t;
+                    t = mul * f2.x; dvx = dvx + t;
+                    t = mul * f2.y; dvy = dvy + t;
+                    t = mul * f1; dvz = dvz + t;
+                }
+                const float vx = V[k][0] + dvx, vy = V[k][1] + dvy, vz = V[k][2] + dvz;
+                vel[3 * j] = vx;
+                vel[3 * j + 1] = vy;
+                vel[3 * j + 2] = vz;
+                if (do_drift) {
+                    drift_one(p, vx, vy, vz, pf, ng);
+                    pos[j] = p;
+                }
             }
-        }
-        float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            float w = gx[c >> 2] * gy[(c >> 1) & 1];
-            w = w * gz[c & 1];
-            const float mul = (float)(((double)w * deltaT) * fscal);
-            float t;
-            t = mul * f2[c].x; dvx = dvx + t;
-            t = mul * f2[c].y; dvy = dvy + t;
-            t = mul * f1[c]; dvz = dvz + t;
-        }
-        const float vx = V[k][0] + dvx, vy = V[k][1] + dvy, vz = V[k][2] + dvz;
-        vel[3 * j] = vx;
-        vel[3 * j + 1] = vy;
-        vel[3 * j + 2] = vz;
-        if (do_drift) {
-            drift_one(p, vx, vy, vz, pf, ng);
-            pos[j] = p;
+            __syncthreads();
         }
     }
 }
@@ -449,18 +713,32 @@ void launch_drift(PosSplitH* pos, const float* vel, int64_t n, int ng, float pf,
     if (!n) return;
     k_drift<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, vel, n, ng, pf);
 }
-void launch_deposit(const PosSplitH* pos, int64_t n, int ng, float mass, long long* rho,
-                    unsigned long long* total, hipStream_t s) {
-    // rho and total must be zero (the caller clears them with launch_zero_u64)
+void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
+                           hipStream_t s) {
     if (!n) return;
-    unsigned nb = (unsigned)((n + DEP_TILE - 1) / DEP_TILE);
-    k_deposit_tiled<<<nb, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho, total);
+    const unsigned nt = (unsigned)(n / DEP_TILE);
+    k_deposit_windows<<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap, (uint4*)b.slots,
+                                            b.count, b.nslots, b.deferred, b.ctl, b.tile_total);
+}
+void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
+    const unsigned nb = (unsigned)(ng / OB);
+    k_deposit_owner<<<nb * nb * nb, OWN_NT, 0, s>>>(b.arena, (const uint4*)b.slots, b.count, b.nslots, ng,
+                                                    (unsigned long long*)rho);
+}
+void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass, long long* rho,
+                             const DepositBuffers& b, hipStream_t s) {
+    if (!n) return;
+    const int64_t nt = n / DEP_TILE;
+    const unsigned grid = (unsigned)(nt < 512 ? nt : 512);
+    k_deposit_deferred<<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho, b.ctl,
+                                               b.deferred, b.tile_total);
+    k_deposit_total<<<1, 1024, 0, s>>>(b.tile_total, nt, b.ctl);
 }
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, hipStream_t s) {
     if (!n) return;
     unsigned nb = (unsigned)((n + GT_TILE - 1) / GT_TILE);
-    k_gather_tiled<<<nb, GT_NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf);
+    k_gather_slab<<<nb, GT_NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf);
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {

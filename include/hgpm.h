@@ -233,7 +233,12 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
 /* Tunables: "fft_cols" and "fft_xsolve" select the FFT column-pass and fused
    x-solve shapes (columns per tile x threads: 0 = 4x256, 1 = 4x512, 2 = 8x512,
    3 = 16x512 where the tile fits LDS); "fft_variants" (get only) is the count valid for
-   this grid.  Unknown keys return HGPM_ERR_ARG. */
+   this grid.  Deposit routing (FP32): "deposit_arena" = usable cells of the window
+   arena (< 0 restores the full allocation of 4 cells per grid cell), "deposit_slots"
+   = chunk descriptors per 16^3 owner block (1..64); tiles that do not fit take the
+   atomic route, with identical results.  "deposit_deferred" (get only, synchronises)
+   = tiles that took the atomic route in the last deposit.  Unknown keys return
+   HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
 /* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each

@@ -42,3 +42,19 @@ def rand_split(rng, n, ng, clustered=False):
     off[bad] = 0.0
     cell[bad] = (cell[bad] + 1) % ng
     return cell, off
+
+
+def lattice_split(rng, np_, ng, sigma, outliers=0):
+    """Lattice particles (reference order, spacing ng/np_) displaced by Gaussian offsets of
+    sigma cells, as the Zel'dovich initial conditions and early steps place them; the
+    first `outliers` particles are thrown anywhere in the box."""
+    q = np.indices((np_,) * 3).reshape(3, -1).T.astype(np.float64) * (ng / np_)
+    x = (q + rng.normal(0, sigma, q.shape)) % ng
+    if outliers:
+        x[:outliers] = rng.random((outliers, 3)) * ng
+    cell = np.floor(x).astype(np.int32) % ng
+    off = (x - np.floor(x)).astype(np.float32)
+    bad = off >= 1.0
+    off[bad] = 0.0
+    cell[bad] = (cell[bad] + 1) % ng
+    return cell, off

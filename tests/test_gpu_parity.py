@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import rand_split
+from conftest import lattice_split, rand_split
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,39 @@ def test_deposit_bit_exact(hg, orc, ng, clustered):
         assert np.array_equal(got, ref)
         m.cic()  # deterministic across runs
         assert np.array_equal(m.get_density(), got)
+
+
+@pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64)])
+def test_deposit_routes(hg, orc, ng, np_):
+    """Arena route (windows -> owner blocks), atomic route for deferred tiles, and the
+    mixes forced by a short arena or a short slot list: all bit-identical."""
+    case(f"deposit routes ng={ng} np={np_}")
+    rng = np.random.default_rng(ng * 7 + np_)
+    mass = (ng / np_) ** 3
+    with mk(hg, ng, np_) as m:
+        cell, off = lattice_split(rng, np_, ng, 1.5, outliers=3)
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        ref = orc.cic_split(ng, cell, off, mass)
+        ntiles = m.N // 4096
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        d0 = m.get_tunable("deposit_deferred")
+        assert d0 < ntiles  # most tiles took the arena route
+        for key, val in [("deposit_arena", 0), ("deposit_arena", 3 * 9000), ("deposit_slots", 1),
+                         ("deposit_slots", 4)]:
+            m.set_tunable(key, val)
+            m.cic()
+            assert np.array_equal(m.get_density(), ref), (key, val)
+            d = m.get_tunable("deposit_deferred")
+            if key == "deposit_arena" and val == 0:
+                assert d == ntiles
+            elif val != 4:  # a 4-slot list may or may not overflow
+                assert 0 < d <= ntiles, (key, val, d)
+            m.set_tunable("deposit_arena", -1)
+            m.set_tunable("deposit_slots", 64)
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_deferred") == d0
 
 
 def test_deposit_mass_ratio(hg, orc):
@@ -149,12 +182,16 @@ def test_solve_fp64_path(hg, orc, ng):
 
 
 # --------------------------------------------------------- kick / drift ----
-@pytest.mark.parametrize("ng", [32, 64])
-def test_kick_drift_bit_exact(hg, orc, ng):
-    case(f"kick/drift ng={ng}")
+@pytest.mark.parametrize("ng,np_,dist", [(32, 32, "random"), (64, 64, "random"), (64, 64, "lattice"),
+                                         (128, 64, "lattice")])
+def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
+    case(f"kick/drift ng={ng} np={np_} {dist}")
     rng = np.random.default_rng(ng + 11)
-    with mk(hg, ng) as m:
-        cell, off = rand_split(rng, m.N, ng)
+    with mk(hg, ng, np_) as m:
+        if dist == "random":
+            cell, off = rand_split(rng, m.N, ng)
+        else:
+            cell, off = lattice_split(rng, np_, ng, 2.5, outliers=2)
         vel = rng.normal(0, 3, (m.N, 3)).astype(np.float32)
         m.set_particles_split(cell, off, vel)
         m.cic()
