@@ -15,7 +15,8 @@ Also reported on the same line:
   fp64_plain   : the plain FP64 GPU path (reference-structured kernels in FP64) on the
                  same workload, and the FP32/FP64 speed-up;
   cpu_baseline : the reference algorithm restated on the host CPU (the oracle in the
-                 reference's FP32 rounding) on 64^3 particles x 120 steps, threads stated.
+                 reference's FP32 rounding) on 128^3 particles x 50 steps at the same
+                 resolution, threads stated.
 """
 import argparse
 import json
@@ -90,14 +91,16 @@ def run_engine(hg, p, steps, warmup, timing=True):
     return el, tim, t_ic, dev_bytes, tune
 
 
-def cpu_baseline(hg, threads):
+def cpu_baseline(hg, threads, ng=128):
     """Reference algorithm on the host: the oracle in the reference's FP32 rounding
-    (float3 positions, FP32 CIC, reference calcGreens), 64^3 particles, z=200 -> 0 in
-    120 steps (60 steps would make the reference's first half step a < 0)."""
+    (float3 positions, FP32 CIC, reference calcGreens), on a bounded sample of the same
+    workload: ng^3 particles at the same resolution (box 2000*ng/1024 Mpc/h), z = 50 -> 0
+    in 50 steps."""
     from oracle import oracle as orc
     orc.set_threads(threads)
-    ng, n = 64, 120
-    p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001)
+    n = 50
+    rl = 2000.0 * ng / 1024
+    p = hg.make_params(ng, ng, rl, 50.0, 0.0, n, seed=2)
     ts = hg.Timestepper(p)
     ts.setInitialZ(p.z_ini)
     ts.reverseHalfStep()
@@ -105,14 +108,14 @@ def cpu_baseline(hg, threads):
     a = 1 / (1 + p.z_ini)
     D, _ = hg.growth(p, a)
     _, dD = hg.growth(p, a - ts.deltaT / 2)
-    pos, vel = orc.ic_f64(ng, ng, 128.0, 21082001, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
+    pos, vel = orc.ic_f64(ng, ng, rl, 2, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
     co = hg.schedule_array(p, n, merge=False)
     t0 = time.perf_counter()
     _, _, t_loop = orc.run_ref32(ng, ng, co, pos.astype(np.float32), vel.astype(np.float32))
     el = time.perf_counter() - t0
     return {"value": ng ** 3 * n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"64^3 particles, 64^3 grid, box 128 Mpc/h, z=200->0, {n} steps (reference FP32 "
-                      f"rounding, FP64 FFT), {el:.1f} s"}
+            "sample": f"{ng}^3 particles, {ng}^3 grid, box {rl:g} Mpc/h, z=50->0, {n} steps (reference "
+                      f"FP32 rounding, FP64 FFT), {el:.1f} s"}
 
 
 def main():
@@ -129,7 +132,19 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device twice (a plain copy kernel of known "
                          "size, the reference point for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--only-copy", action="store_true",
+                    help="run only the --copy-calibration copy (a separate profiler pass)")
     args = ap.parse_args()
+    if args.only_copy:
+        import torch
+        n = max(args.copy_calibration, 1) << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs a single GPU (one rank); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
@@ -152,7 +167,10 @@ def main():
     breakdown = {k: {"ms_per_step": v[0] / args.steps, "launches": v[1],
                      "GBps": (kernel_bytes(k, Np, G) / (v[0] / v[1] * 1e-3) / 1e9) if kernel_bytes(k, Np, G) and v[1] else None}
                  for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0])}
-    step_alg = 96 * Np + 136 * G  # sum of the per-kernel algorithmic bytes of one fused step
+    # algorithmic bytes of one step with complex-to-complex transforms: deposit 24 Np + 8 G,
+    # forward FFT 48 G, three inverse FFTs with the solve fused 132 G, gather/kick/drift
+    # 72 Np + 12 G
+    step_alg = 96 * Np + 200 * G
     out = {
         "metric": METRIC,
         "value": value,

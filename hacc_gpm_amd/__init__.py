@@ -420,11 +420,12 @@ class MemoryManager:
     def autotune(self, reps=3):
         """Times the FFT variants on this grid and keeps the fastest; returns the
         per-variant milliseconds (column pass, fused x-solve)."""
-        mc = np.zeros(4)
-        mx = np.zeros(4)
-        self._c(lib().hgpm_autotune(self.h, int(reps), _ptr(mc), _ptr(mx)))
         n = self.get_tunable("fft_variants")
-        return {"cols_ms": mc[:n].tolist(), "xsolve_ms": mx[:n].tolist(),
+        mc = np.zeros(n)
+        mx = np.zeros(n)
+        self._c(lib().hgpm_autotune(self.h, int(reps), _ptr(mc), _ptr(mx)))
+        return {"cols_ms": [v if v >= 0 else None for v in mc.tolist()],
+                "xsolve_ms": [v if v >= 0 else None for v in mx.tolist()],
                 "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve")}
 
     # instrumentation

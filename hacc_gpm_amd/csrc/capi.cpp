@@ -1050,9 +1050,11 @@ int hgpm_get_timings(hgpm_ctx* c, char* names, int names_len, double* total_ms, 
 }
 int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
     if (!c || !key) return HGPM_ERR_ARG;
-    const int nv = c->fft ? c->fft->valid_variants : 0;
+    const int nv = c->fft ? c->fft->n_variants : 0;
+    const unsigned mask = c->fft ? c->fft->valid_mask : 0u;
     if (!strcmp(key, "fft_cols") || !strcmp(key, "fft_xsolve")) {
-        if (value < 0 || value >= nv) return fail(c, HGPM_ERR_ARG, std::string(key) + " variant out of range");
+        if (value < 0 || value >= nv || !((mask >> value) & 1u))
+            return fail(c, HGPM_ERR_ARG, std::string(key) + " variant not available for this grid");
         (!strcmp(key, "fft_cols") ? c->var_cols : c->var_xs) = value;
         return HGPM_OK;
     }
@@ -1073,7 +1075,8 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     if (!c || !key || !value) return HGPM_ERR_ARG;
     if (!strcmp(key, "fft_cols")) { *value = c->var_cols; return HGPM_OK; }
     if (!strcmp(key, "fft_xsolve")) { *value = c->var_xs; return HGPM_OK; }
-    if (!strcmp(key, "fft_variants")) { *value = c->fft ? c->fft->valid_variants : 0; return HGPM_OK; }
+    if (!strcmp(key, "fft_variants")) { *value = c->fft ? c->fft->n_variants : 0; return HGPM_OK; }
+    if (!strcmp(key, "fft_variant_mask")) { *value = c->fft ? (int)c->fft->valid_mask : 0; return HGPM_OK; }
     if (!strcmp(key, "deposit_arena")) {
         *value = (int)std::min<unsigned long long>(c->arena_cap, 0x7fffffffull);
         return HGPM_OK;
@@ -1103,7 +1106,12 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     launch_zero_u64((unsigned long long*)c->A, (size_t)c->G, c->st);
     launch_zero_u64((unsigned long long*)c->B, (size_t)c->G, c->st);
     double best_c = 1e30, best_x = 1e30;
-    for (int v = 0; v < f->valid_variants; v++) {
+    for (int v = 0; v < f->n_variants; v++) {
+        if (!((f->valid_mask >> v) & 1u)) {
+            if (ms_cols) ms_cols[v] = -1.0;
+            if (ms_xsolve) ms_xsolve[v] = -1.0;
+            continue;
+        }
         for (int kind = 0; kind < 2; kind++) {
             auto run = [&]() {
                 if (kind == 0) f->pass_f32(c->A, 1, -1, c->twf, v, c->st);

@@ -10,8 +10,8 @@ namespace hgpm {
 struct FftOps {
     int n;
     int stages;
-    int n_variants;      // column-pass / x_solve variants compiled (ids 0..n_variants-1)
-    int valid_variants;  // variants usable for this length
+    int n_variants;      // column-pass / x_solve variant ids 0..n_variants-1
+    unsigned valid_mask; // bit v set: variant v usable for this length
     void (*twiddles)(double* fwd, double* rev); // stages*n complex each
     // axis 2 = rows (z), 1 = y, 0 = x; variant selects the column-pass shape
     void (*pass_f32)(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s);

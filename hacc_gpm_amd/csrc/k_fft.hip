@@ -142,8 +142,10 @@ __global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* _
 
 // --------------------------------------------------------- launchers ----
 // Column-pass / x_solve variants (columns per tile, threads per workgroup):
-//   0: (4, 256)  1: (4, 512)  2: (8, 512)  3: (16, 512)
-// chosen per transform length by the autotuner (default below).
+//   0: (4, 256)  1: (4, 512)  2: (8, 512)  3: (16, 512)  4: (8, 1024)  5: (16, 1024)
+// chosen per transform length by the autotuner (default 2).  More threads per tile
+// means fewer elements, and registers, per thread: the fused x-solve holds the
+// spectrum between its three transforms and needs them.
 template <typename V, int CC, int NTC>
 void cols_launch(V* d, int axis, int dir, const V* tw, hipStream_t s) {
     unsigned nb = (unsigned)((size_t)N * (N / CC));
@@ -156,19 +158,36 @@ void rows_launch(V* d, int dir, const V* tw, hipStream_t s) {
     if (dir < 0) k_rows_c2c<V, RC, -1><<<nb, NT, 0, s>>>(d, tw);
     else k_rows_c2c<V, RC, +1><<<nb, NT, 0, s>>>(d, tw);
 }
-constexpr int cc_of(int v) { return v == 0 ? 4 : (v == 1 ? 4 : (v == 2 ? 8 : 16)); }
-constexpr int nt_of(int v) { return v == 0 ? 256 : 512; }
+constexpr int NVAR = 6;
+constexpr int cc_of(int v) { return v <= 1 ? 4 : (v == 2 || v == 4 ? 8 : 16); }
+constexpr int nt_of(int v) { return v == 0 ? 256 : (v <= 3 ? 512 : 1024); }
 constexpr bool variant_ok(int v) { return cc_of(v) <= N && (N % cc_of(v)) == 0 && (size_t)N * cc_of(v) * 8 <= 140000; }
+constexpr unsigned variant_mask() {
+    unsigned m = 0;
+    for (int v = 0; v < NVAR; v++)
+        if (variant_ok(v)) m |= 1u << v;
+    return m;
+}
+static_assert(variant_ok(2), "default variant");
 
+// Calls f(IntC<v>) for the runtime variant v when it is valid for N, else variant 2.
+template <typename F> void with_variant(int variant, F&& f) {
+    bool done = false;
+    auto one = [&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        if constexpr (variant_ok(v)) {
+            if (!done && variant == v) { f(vc); done = true; }
+        }
+    };
+    one(IntC<0>{}); one(IntC<1>{}); one(IntC<2>{}); one(IntC<3>{}); one(IntC<4>{}); one(IntC<5>{});
+    if (!done) f(IntC<2>{});
+}
 void pass_f32(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s) {
     if (axis == 2) { rows_launch<float2, rc32()>(d, dir, tw, s); return; }
-    switch (variant) {
-    case 0: cols_launch<float2, cc_of(0), nt_of(0)>(d, axis, dir, tw, s); break;
-    case 1: cols_launch<float2, cc_of(1), nt_of(1)>(d, axis, dir, tw, s); break;
-    case 3: if constexpr (variant_ok(3)) { cols_launch<float2, cc_of(3), nt_of(3)>(d, axis, dir, tw, s); break; }
-            [[fallthrough]];
-    default: cols_launch<float2, cc_of(2), nt_of(2)>(d, axis, dir, tw, s); break;
-    }
+    with_variant(variant, [&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, s);
+    });
 }
 void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
     if (axis == 2) rows_launch<double2, rc64()>(d, dir, tw, s);
@@ -192,13 +211,10 @@ void x_solve_launch(float2* a, float2* p, const float2* twf, const float2* twr, 
 }
 void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
              int variant, hipStream_t s) {
-    switch (variant) {
-    case 0: x_solve_launch<cc_of(0), nt_of(0)>(a, p, twf, twr, sk, hs, s); break;
-    case 1: x_solve_launch<cc_of(1), nt_of(1)>(a, p, twf, twr, sk, hs, s); break;
-    case 3: if constexpr (variant_ok(3)) { x_solve_launch<cc_of(3), nt_of(3)>(a, p, twf, twr, sk, hs, s); break; }
-            [[fallthrough]];
-    default: x_solve_launch<cc_of(2), nt_of(2)>(a, p, twf, twr, sk, hs, s); break;
-    }
+    with_variant(variant, [&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        x_solve_launch<cc_of(v), nt_of(v)>(a, p, twf, twr, sk, hs, s);
+    });
 }
 void twiddles(double* fwd, double* rev) {
     make_twiddles<N, false>(fwd);
@@ -212,7 +228,7 @@ namespace hgpm {
 #define HGPM_CAT2(a, b) a##b
 #define HGPM_CAT(a, b) HGPM_CAT2(a, b)
 extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
-    N, Plan<N>::S, 4, variant_ok(3) ? 4 : 3, twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
+    N, Plan<N>::S, NVAR, variant_mask(), twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
 };
 } // namespace hgpm
 #endif

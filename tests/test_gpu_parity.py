@@ -148,7 +148,8 @@ def test_fft_variants_agree(hg, orc, ng):
     rng = np.random.default_rng(ng + 21)
     with mk(hg, ng) as m:
         nv = m.get_tunable("fft_variants")
-        assert nv >= 3
+        mask = m.get_tunable("fft_variant_mask")
+        assert nv == 6 and mask & 0b100
         cell, off = rand_split(rng, m.N, ng, clustered=True)
         m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
         m.cic()
@@ -156,6 +157,10 @@ def test_fft_variants_agree(hg, orc, ng):
         Fo = orc.solve_gradient(ng, q * 2.0 ** -24)
         x = (rng.standard_normal((ng,) * 3) + 1j * rng.standard_normal((ng,) * 3)).astype(np.complex64)
         for v in range(nv):
+            if not (mask >> v) & 1:
+                with pytest.raises(hg.HgpmError):
+                    m.set_tunable("fft_cols", v)
+                continue
             m.set_tunable("fft_cols", v)
             m.set_tunable("fft_xsolve", v)
             m.set_density(q)
@@ -165,7 +170,8 @@ def test_fft_variants_agree(hg, orc, ng):
             assert err < 2e-5, (v, err)
             assert rel_l2(m.fft(x, -1), np.fft.fftn(x.astype(np.complex128))) < 2e-6, v
         res = m.autotune(2)
-        assert 0 <= res["fft_cols"] < nv and 0 <= res["fft_xsolve"] < nv
+        assert (mask >> res["fft_cols"]) & 1 and (mask >> res["fft_xsolve"]) & 1
+        assert all((t is None) == (not (mask >> v) & 1) for v, t in enumerate(res["cols_ms"]))
 
 
 @pytest.mark.parametrize("ng", [32, 64])

@@ -248,6 +248,39 @@ def test_growth_against_quad(orc):
         assert abs(D - e["a"]) < 1e-10 * e["a"] and abs(dD - 1) < 1e-9
 
 
+def test_ic_displacement_is_the_potential_flow_of_the_scaled_noise(orc):
+    """Zel'dovich IC: for a power-law table (exact under log-log interpolation) the
+    displacement must be psi_d^(k) = -i k_d delta^(k) / k^2 with delta^ = FFT(noise) *
+    sqrt(P(|k|) N / rl^3), checked with numpy's FFT mode by mode (Nyquist planes
+    excluded: the real part taken there breaks Hermitian symmetry), and the velocity
+    must be one constant times the displacement."""
+    n, rl, seed = 16, 32.0, 7
+    A, ns, kd = 3.0, -1.5, 0.01
+    table = A * (kd * np.arange(1, 4001)) ** ns
+    table = np.concatenate([[0.0], table])
+    pos, vel = orc.ic_f64(n, n, rl, seed, table, kd, 0.7, 1.3, 50.0, 2.1, 0.01)
+    q = np.indices((n,) * 3).reshape(3, -1).T.astype(np.float64)
+    psi = pos - q
+    psi -= n * np.round(psi / n)
+    noise = orc.white_noise(seed, n ** 3).reshape((n,) * 3)
+    m = np.fft.fftfreq(n, 1.0 / n)
+    m[m == n // 2] = -n // 2
+    k1 = 2 * np.pi / rl * m
+    KX, KY, KZ = np.meshgrid(k1, k1, k1, indexing="ij")
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    P = np.where(k2 > 0, A * np.sqrt(np.where(k2 > 0, k2, 1.0)) ** ns, 0.0) * n ** 3 / rl ** 3
+    dhat = np.fft.fftn(noise) * np.sqrt(P)
+    inner = (np.abs(m) < n // 2)
+    sel = inner[:, None, None] & inner[None, :, None] & inner[None, None, :]
+    for d, K in enumerate((KX, KY, KZ)):
+        got = np.fft.fftn(psi[:, d].reshape((n,) * 3))
+        want = np.where(k2 > 0, -1j * K * dhat / np.where(k2 > 0, k2, 1.0), 0.0)
+        err = np.abs(got - want)[sel].max() / np.abs(want[sel]).max()
+        assert err < 1e-6, (d, err)
+    c = float(np.sum(vel * psi) / np.sum(psi * psi))
+    assert np.abs(vel - c * psi).max() < 1e-12 * np.abs(vel).max() + 1e-300
+
+
 def test_reference_float_growth_close_to_fp64(orc):
     # the reference's 100-bin FP32 trapezoid agrees with the FP64 growth to ~1e-3
     om = 0.31
