@@ -41,6 +41,8 @@ def kernel_bytes(name, Np, G):
         "drift": 60 * Np,                      # read pos 24 + vel 12, write pos 24
         "zero_density": 8 * G,                 # write int64 density
         "deposit": 24 * Np + 8 * G,            # read pos, write density
+        "deposit_windows": 24 * Np,            # read pos (the window arena is overhead)
+        "deposit_owner": 8 * G,                # write density
         "fft_fwd_z_density": 16 * G,           # read int64, write complex64
         "fft_fwd_y": 16 * G,                   # read + write complex64
         "fft_x_solve": 24 * G,                 # read rho^, write P and Q
@@ -51,6 +53,23 @@ def kernel_bytes(name, Np, G):
         "gather_kick": 48 * Np + 12 * G,
     }
     return table.get(name)
+
+
+def measured_traffic(workload, region):
+    """HBM bytes per launch of a timed region, from the committed counter summaries
+    (profiles/*.md lines `traffic: <workload> <region> <bytes>`; FETCH_SIZE and
+    WRITE_SIZE passes scaled by a plain copy of known size).  None if not measured."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for fn in sorted(os.listdir(d)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(d, fn)):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic:" and f[1] == workload and f[2] == region:
+                return float(f[3])
+    return None
 
 
 def make_ics(hg, p, ts):
@@ -151,6 +170,7 @@ def main():
     import hacc_gpm_amd as hg
 
     ng = args.ng
+    workload = f"pm_step_{ng}^3_particles_{ng}^3_grid_box{int(args.rl)}"
     p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221)
     Np = ng ** 3
     G = ng ** 3
@@ -184,12 +204,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: Zel'dovich ICs at z=50 from Philox white noise and an Eisenstein-Hu no-wiggle P(k)",
-        "config": {"workload": f"pm_step_{ng}^3_particles_{ng}^3_grid_box{int(args.rl)}", "np": ng, "ng": ng,
+        "config": {"workload": workload, "np": ng, "ng": ng,
                    "rl": args.rl, "z_ini": 50, "steps_in_run": 50, "positions": "int32 cell + f32 offset",
                    "deposit": "int64 fixed point 2^-24"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK if achieved else None,
+                     "traffic": measured_traffic(workload, name),
                      "alg_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / HBM_PEAK},
