@@ -45,6 +45,7 @@ struct hgpm_ctx {
     float2* A = nullptr;             // spectrum / Fz^ (8 B per cell)
     float2* B = nullptr;             // int64 density / Fx^+iFy^ / (Fx,Fy) (8 B per cell)
     float* fz = nullptr;             // Fz (4 B per cell)
+    float2 *h2 = nullptr, *h3 = nullptr; // half spectra Fx^, Fy^ of the solve (A holds rho^ / Fz^)
     unsigned long long* ctl = nullptr;   // deposit control words; ctl[0] = deposited total
     unsigned long long* total = nullptr; // = ctl
     // deposit arena route (k_particles.hip): window arena, per-owner-block chunk slots
@@ -78,6 +79,7 @@ struct hgpm_ctx {
     // tunables: column-pass and x_solve variants of the FFT (see k_fft.hip), set by
     // hgpm_set_tunable or hgpm_autotune
     int var_cols = 2, var_xs = 2;
+    int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
 };
 
 namespace {
@@ -248,32 +250,35 @@ int f32_deposit(hgpm_ctx* c) {
     return HGPM_OK;
 }
 
+// Poisson solve and gradient on half spectra: density (B, int64) -> rho^ (A, pitch hz)
+// -> Fx^, Fy^, Fz^ (h2, h3, A) -> (Fx, Fy) float2 in B and Fz in fz.
 int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {
-        Timed t(c, "fft_fwd_z_density");
-        f->fwd_z_density((const long long*)c->B, c->total, c->A, c->twf, c->st);
+        Timed t(c, "fft_fwd_z_r2c");
+        f->fwd_z_r2c((const long long*)c->B, c->total, c->A, c->twf, c->st);
     }
     {
         Timed t(c, "fft_fwd_y");
-        f->pass_f32(c->A, 1, -1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->A, 1, -1, c->twf, c->var_cols, c->st);
     }
     {
         Timed t(c, "fft_x_solve");
-        f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->st);
+        f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->st);
     }
     {
         Timed t(c, "fft_inv_y");
-        f->pass_f32(c->B, 1, +1, c->twf, c->var_cols, c->st);
-        f->pass_f32(c->A, 1, +1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->h2, 1, +1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->h3, 1, +1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->A, 1, +1, c->twf, c->var_cols, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fxy");
-        f->pass_f32(c->B, 2, +1, c->twf, c->var_cols, c->st);
+        f->inv_z_c2r_xy(c->h2, c->h3, c->B, c->twf, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fz");
-        f->inv_z_real(c->A, c->fz, c->twf, c->st);
+        f->inv_z_c2r_pair(c->A, c->fz, c->twf, c->st);
     }
     LAUNCHCHK(c);
     return HGPM_OK;
@@ -411,6 +416,8 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
             (rc = dalloc(c, &c->A, sizeof(float2) * G, "grid A")) ||
             (rc = dalloc(c, &c->B, sizeof(float2) * G, "grid B")) ||
             (rc = dalloc(c, &c->fz, sizeof(float) * G, "Fz")) ||
+            (rc = dalloc(c, &c->h2, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz, "half spectrum Fx")) ||
+            (rc = dalloc(c, &c->h3, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz, "half spectrum Fy")) ||
             (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")))
             return bail(rc);
         c->total = c->ctl;
@@ -493,7 +500,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->twf, c->twr, c->sk, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->twf, c->twr, c->sk, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -855,7 +862,7 @@ int hgpm_update_velocities(hgpm_ctx* c, double deltaT, double fscal) {
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick");
-        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->st);
+        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather, c->st);
     } else {
         Timed t(c, "f64_icic");
         launch_f64_icic(c->pos64, c->vel64, c->grad64, c->N, c->ng, deltaT, fscal, c->st);
@@ -871,7 +878,8 @@ int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) 
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick_drift");
-        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor, c->st);
+        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor, c->var_gather,
+                           c->st);
         LAUNCHCHK(c);
         return HGPM_OK;
     }
@@ -1063,6 +1071,11 @@ int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
         c->arena_cap = value < 0 ? c->arena_alloc : std::min<unsigned long long>((unsigned long long)value, c->arena_alloc);
         return HGPM_OK;
     }
+    if (!strcmp(key, "gather_tile")) {
+        if (value < 0 || value > 1) return fail(c, HGPM_ERR_ARG, "gather_tile must be 0 or 1");
+        c->var_gather = value;
+        return HGPM_OK;
+    }
     if (!strcmp(key, "deposit_slots")) {
         if (value < 1 || value > HGPM_DEP_SLOTS)
             return fail(c, HGPM_ERR_ARG, "deposit_slots must be in [1," + std::to_string(HGPM_DEP_SLOTS) + "]");
@@ -1082,6 +1095,7 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
         return HGPM_OK;
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
+    if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
     if (!strcmp(key, "deposit_deferred")) { // tiles that took the atomic route in the last deposit
         if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
         unsigned long long v[2];
@@ -1114,8 +1128,8 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
         }
         for (int kind = 0; kind < 2; kind++) {
             auto run = [&]() {
-                if (kind == 0) f->pass_f32(c->A, 1, -1, c->twf, v, c->st);
-                else f->x_solve(c->A, c->B, c->twf, c->twr, c->sk, c->hs, v, c->st);
+                if (kind == 0) f->pass_half(c->A, 1, -1, c->twf, v, c->st);
+                else f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, v, c->st);
             };
             run();
             HIPCHK(c, hipEventRecord(a, c->st));

@@ -18,9 +18,15 @@ struct FftOps {
     void (*pass_f64)(double2* d, int axis, int dir, const double2* tw, hipStream_t s);
     void (*fwd_z_density)(const long long* rho, const unsigned long long* total, float2* out,
                           const float2* tw, hipStream_t s);
-    void (*inv_z_real)(const float2* in, float* out, const float2* tw, hipStream_t s);
-    void (*x_solve)(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk,
-                    const float* hs, int variant, hipStream_t s);
+    // half-spectrum solve (rows of hz = n/2 + 16 complex, k_z = 0..n/2)
+    int hz;
+    void (*pass_half)(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s);
+    void (*fwd_z_r2c)(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
+                      hipStream_t s);
+    void (*x_solve_r2c)(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr,
+                        const float* sk, const float* hs, int variant, hipStream_t s);
+    void (*inv_z_c2r_xy)(const float2* hx, const float2* hy, float2* fxy, const float2* tw, hipStream_t s);
+    void (*inv_z_c2r_pair)(const float2* hz, float* fz, const float2* tw, hipStream_t s);
 };
 
 #define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
@@ -69,8 +75,9 @@ void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mas
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s);
 void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s);
+// variant 0: 512-thread workgroups on half tiles; 1: 256-thread workgroups on quarter tiles
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, hipStream_t s);
+                        double deltaT, double fscal, int do_drift, float pf, int variant, hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
                               int ng, hipStream_t s);

@@ -4,13 +4,19 @@
 // x-major ng^3 grids) and fuses the spectral solve of src/pm_solver/solver.cpp:145-170
 // into the transform passes.
 //
-// Passes (grid index (x*N + y)*N + z, z contiguous):
+// Passes (grid index (x*N + y)*PZ + z, z contiguous; PZ = N for full complex grids):
 //   rows  : 1-D transforms along z; a workgroup loads RC whole rows (coalesced).
-//   cols  : 1-D transforms along y (stride N) or x (stride N^2); a workgroup loads CC
+//   cols  : 1-D transforms along y (stride PZ) or x (stride N*PZ); a workgroup loads CC
 //           adjacent z values of N rows (CC*8 contiguous bytes per row).
-//   x_solve (FP32): forward transform along x, Green's function x central-difference
-//           gradient for all three axes, and the inverse transforms along x of
-//           P = Fx^ + i Fy^ and Q = Fz^, without the spectrum leaving registers.
+// The solve of a step runs on half spectra: the density and the three force
+// components are real, so only k_z = 0..N/2 is kept (row pitch HZ = N/2 + 16, the
+// padding columns held at zero) and every pass after the first touches half the data:
+//   r2c   : pairs of real rows a, b as one complex transform of a + i b, split into
+//           the half spectra of a and b;
+//   x_solve_r2c: forward x transform, then the inverse x transforms of
+//           Fx^ = i g sin k_x rho^, Fy^ = i g sin k_y rho^, Fz^ = i g sin k_z rho^;
+//   c2r   : (Fx, Fy) as one complex transform of the Hermitian extension of
+//           Fx^ + i Fy^; Fz rows in pairs the same way.
 #include "fft.h"
 #include "internal.h"
 
@@ -21,6 +27,8 @@
 namespace {
 constexpr int N = HGPM_FFT_N;
 constexpr int NT = 256;   // rows passes
+constexpr int HZ = N / 2 + 16; // half-spectrum row pitch: k_z = 0..N/2 + zero padding
+static_assert(HZ % 16 == 0 && HZ <= N, "half-spectrum pitch");
 
 // rows per row tile / columns per column tile (FP32, FP64)
 constexpr int rc32() { return N >= 768 ? 4 : (N >= 384 ? 8 : (N >= 192 ? 16 : (N >= 96 ? 32 : 64))); }
@@ -55,59 +63,83 @@ __global__ __launch_bounds__(NT) void k_rows_density(const long long* __restrict
         [&](int, int, int n, int c, float2 v) { out[(row0 + c) * N + n] = v; }, lds, tw);
 }
 
-// Inverse transform along z keeping the real part (Fz).
-template <int RC>
-__global__ __launch_bounds__(NT) void k_rows_real_out(const float2* __restrict in, float* __restrict out,
-                                                      const float2* __restrict tw) {
-    __shared__ float2 lds[lds_elems<N, RC>()];
-    const size_t row0 = (size_t)blockIdx.x * RC;
-    fft_tile<float2, N, RC, NT, +1, false, true>(
-        [&](int, int, int n, int c) { return in[(row0 + c) * N + n]; },
-        [&](int, int, int n, int c, float2 v) { out[(row0 + c) * N + n] = v.x; }, lds, tw);
-}
-
 // ------------------------------------------------------------- cols ----
 // axis 1 (y): tile = (x, z-chunk), element (n=y, c) at x N^2 + n N + z0 + c
 // axis 0 (x): tile = (y, z-chunk), element (n=x, c) at n N^2 + y N + z0 + c
 template <typename V, int CC, int NTC, int DIR>
-__global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis) {
+__global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis, int pz) {
     __shared__ V lds[lds_elems<N, CC>()];
     // adjacent z-chunks of one row share 128-byte lines: keep them on one XCD's L2
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int outer = tile / (N / CC), zc = tile % (N / CC);
-    const size_t NN = (size_t)N * N;
-    const size_t base = axis == 1 ? (size_t)outer * NN + (size_t)zc * CC : (size_t)outer * N + (size_t)zc * CC;
-    const size_t stride = axis == 1 ? (size_t)N : NN;
+    const int nzc = pz / CC;
+    const int outer = tile / nzc, zc = tile % nzc;
+    const size_t NP = (size_t)N * pz;
+    const size_t base = axis == 1 ? (size_t)outer * NP + (size_t)zc * CC : (size_t)outer * pz + (size_t)zc * CC;
+    const size_t stride = axis == 1 ? (size_t)pz : NP;
     fft_tile<V, N, CC, NTC, DIR, false, false>(
         [&](int, int, int n, int c) { return d[base + n * stride + c]; },
         [&](int, int, int n, int c, V v) { d[base + n * stride + c] = v; }, lds, tw);
 }
 
-// ---------------------------------------------------------- x_solve ----
-// In: a = rho^ after the z and y forward passes.  Out: p = IFFT_x(Fx^ + i Fy^),
-// a = IFFT_x(Fz^).  F_d^ = -i sin(k_d) G rho^ with
-// G = (0.5/N^3)/(sum_d cos k_d - 3) = -(0.25/N^3)/sum_d sin^2(k_d/2)
-// (kspace_solve_gradient, src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens,
-// src/pm_solver/cuda/calcGreens.hpp:11-26), G(0) = 0.  sk[m] = sin(2 pi m_s/N) with
-// the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274); hs[m] = sin^2(pi m/N).
+// ------------------------------------------------- half-spectrum solve ----
+// Forward z transform of the int64 density (mean removed in FP64, as k_rows_density)
+// for RC real rows: rows 2c and 2c+1 form the complex column c = a + i b, transformed
+// once; A[k] = (C[k] + conj C[N-k]) / 2 and B[k] = (C[k] - conj C[N-k]) / 2i give the
+// half spectra, k = 0..N/2, written with pitch HZ (padding written as zero).
+template <int RC>
+__global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __restrict rho,
+                                                         const unsigned long long* __restrict total,
+                                                         float2* __restrict out, const float2* __restrict tw) {
+    constexpr int PC = RC / 2;
+    __shared__ float2 lds[lds_elems<N, PC>()];
+    float2* spec = lds; // the last stage has read all its inputs before the first output
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    const double mean = (double)(long long)(*total) / ((double)N * N * N);
+    fft_tile<float2, N, PC, NT, -1, false, true>(
+        [&](int, int, int n, int c) {
+            const double qa = (double)rho[(row0 + 2 * c) * N + n];
+            const double qb = (double)rho[(row0 + 2 * c + 1) * N + n];
+            return make_float2((float)((qa - mean) * (1.0 / 16777216.0)), (float)((qb - mean) * (1.0 / 16777216.0)));
+        },
+        [&](int, int, int n, int c, float2 v) { spec[lds_slot<N, PC, true>(n, c)] = v; }, lds, tw);
+    __syncthreads();
+    for (int i = threadIdx.x; i < PC * HZ; i += NT) {
+        const int c = i / HZ, k = i % HZ;
+        float2 a = make_float2(0.0f, 0.0f), b = make_float2(0.0f, 0.0f);
+        if (k <= N / 2) {
+            const float2 x = spec[lds_slot<N, PC, true>(k, c)];
+            const float2 y = spec[lds_slot<N, PC, true>(k == 0 ? 0 : N - k, c)];
+            a = make_float2(0.5f * (x.x + y.x), 0.5f * (x.y - y.y));
+            b = make_float2(0.5f * (x.y + y.y), 0.5f * (y.x - x.x));
+        }
+        out[(row0 + 2 * c) * HZ + k] = a;
+        out[(row0 + 2 * c + 1) * HZ + k] = b;
+    }
+}
+
+// Half-spectrum x_solve.  In: h1 = rho^ after the z and y forward passes.  Forward x
+// transform, then the inverse x transforms of F_d^ = -i sin(k_d) G rho^ =
+// i g sin(k_d) rho^ with G = (0.5/N^3)/(sum_d cos k_d - 3) = -g,
+// g = (0.25/N^3)/sum_d sin^2(k_d/2), G(0) = 0 (kspace_solve_gradient,
+// src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens, src/pm_solver/cuda/calcGreens.hpp:11-26):
+// Fx^ -> h2, Fy^ -> h3, Fz^ -> h1, without the spectrum leaving registers.  sk[m] =
+// sin(2 pi m_s/N) with the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274);
+// hs[m] = sin^2(pi m/N).
 template <int CC, int NTC>
-__global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* __restrict p,
-                                                 const float2* __restrict twf, const float2* __restrict twr,
-                                                 const float* __restrict sk, const float* __restrict hs,
-                                                 float gcoef) {
+__global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, float2* __restrict h2,
+                                                     float2* __restrict h3, const float2* __restrict twf,
+                                                     const float2* __restrict twr, const float* __restrict sk,
+                                                     const float* __restrict hs, float gcoef) {
     __shared__ float2 lds[lds_elems<N, CC>()];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int y = tile / (N / CC), zc = tile % (N / CC);
-    const size_t NN = (size_t)N * N;
-    const size_t base = (size_t)y * N + (size_t)zc * CC;
+    const int y = tile / (HZ / CC), zc = tile % (HZ / CC);
+    const size_t NP = (size_t)N * HZ;
+    const size_t base = (size_t)y * HZ + (size_t)zc * CC;
     typedef LastShape<N, CC, NTC> LS;
     typedef StageShape<N, CC, NTC, LS::R> SS;
-    float2 R[LS::NB][LS::R]; // rho^ of this thread's elements, between the transforms
-    // The factors this thread needs, loaded before the forward transform so their
-    // latency hides behind it: its column c (fixed: NTC is a multiple of CC) gives z,
-    // its butterflies give the x modes n = j + r*N/R.
+    float2 R[LS::NB][LS::R];
     const int c0 = threadIdx.x % CC;
-    const int z = zc * CC + c0;
+    const int z = zc * CC + c0; // < HZ <= N
     const float sy = sk[y], sz = sk[z];
     float G[LS::NB][LS::R], SX[LS::NB][LS::R];
 #pragma unroll
@@ -116,28 +148,74 @@ __global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* _
         for (int r = 0; r < LS::R; r++) {
             const int w = threadIdx.x + b * NTC;
             const int n = (SS::TOT % NTC == 0 || w < SS::TOT) ? (w / CC + r * SS::NBF) : 0;
-            // g = (0.25/N^3) / sum_d sin^2(k_d/2) = -G (0 at k = 0)
             const float ssum = (hs[n] + hs[y]) + hs[z];
             G[b][r] = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
             SX[b][r] = sk[n];
         }
     fft_tile<float2, N, CC, NTC, -1, false, false>(
-        [&](int, int, int n, int c) { return a[base + n * NN + c]; },
+        [&](int, int, int n, int c) { return h1[base + n * NP + c]; },
         [&](int b, int r, int, int, float2 v) { R[b][r] = v; }, lds, twf);
-    // P = Fx^ + i Fy^ = g (-sin k_y + i sin k_x) rho^
+    // i g s rho^ = g s (-rho.y, rho.x)
     fft_tile<float2, N, CC, NTC, +1, true, false>(
         [&](int b, int r, int, int) {
-            const float gx = G[b][r] * R[b][r].x, gy = G[b][r] * R[b][r].y, sx = SX[b][r];
-            return make_float2(-sy * gx - sx * gy, -sy * gy + sx * gx);
+            const float f = G[b][r] * SX[b][r];
+            return make_float2(-f * R[b][r].y, f * R[b][r].x);
         },
-        [&](int, int, int n, int c, float2 v) { p[base + n * NN + c] = v; }, lds, twr);
-    // Q = Fz^ = i g sin k_z rho^
+        [&](int, int, int n, int c, float2 v) { h2[base + n * NP + c] = v; }, lds, twr);
     fft_tile<float2, N, CC, NTC, +1, true, false>(
         [&](int b, int r, int, int) {
-            const float gx = G[b][r] * R[b][r].x, gy = G[b][r] * R[b][r].y;
-            return make_float2(-sz * gy, sz * gx);
+            const float f = G[b][r] * sy;
+            return make_float2(-f * R[b][r].y, f * R[b][r].x);
         },
-        [&](int, int, int n, int c, float2 v) { a[base + n * NN + c] = v; }, lds, twr);
+        [&](int, int, int n, int c, float2 v) { h3[base + n * NP + c] = v; }, lds, twr);
+    fft_tile<float2, N, CC, NTC, +1, true, false>(
+        [&](int b, int r, int, int) {
+            const float f = G[b][r] * sz;
+            return make_float2(-f * R[b][r].y, f * R[b][r].x);
+        },
+        [&](int, int, int n, int c, float2 v) { h1[base + n * NP + c] = v; }, lds, twr);
+}
+
+// Element n of the Hermitian extension of a half-spectrum row: h[n] for n <= N/2,
+// conj h[N-n] above.
+__device__ __forceinline__ float2 herm(const float2* __restrict row, int n) {
+    if (n <= N / 2) return row[n];
+    const float2 v = row[N - n];
+    return make_float2(v.x, -v.y);
+}
+
+// Inverse z transform of (Fx^, Fy^) half spectra: C = Fx^ + i Fy^ over all k_z, one
+// complex transform per row; its real and imaginary parts are Fx and Fy (float2 out).
+template <int RC>
+__global__ __launch_bounds__(NT) void k_rows_c2r_xy(const float2* __restrict hx, const float2* __restrict hy,
+                                                    float2* __restrict fxy, const float2* __restrict tw) {
+    __shared__ float2 lds[lds_elems<N, RC>()];
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    fft_tile<float2, N, RC, NT, +1, false, true>(
+        [&](int, int, int n, int c) {
+            const float2 a = herm(hx + (row0 + c) * HZ, n), b = herm(hy + (row0 + c) * HZ, n);
+            return make_float2(a.x - b.y, a.y + b.x);
+        },
+        [&](int, int, int n, int c, float2 v) { fxy[(row0 + c) * N + n] = v; }, lds, tw);
+}
+
+// Inverse z transform of Fz^ half spectra, rows in pairs: C = Fz^_a + i Fz^_b.
+template <int RC>
+__global__ __launch_bounds__(NT) void k_rows_c2r_pair(const float2* __restrict hz, float* __restrict fz,
+                                                      const float2* __restrict tw) {
+    constexpr int PC = RC / 2;
+    __shared__ float2 lds[lds_elems<N, PC>()];
+    const size_t row0 = (size_t)blockIdx.x * RC;
+    fft_tile<float2, N, PC, NT, +1, false, true>(
+        [&](int, int, int n, int c) {
+            const float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
+            return make_float2(a.x - b.y, a.y + b.x);
+        },
+        [&](int, int, int n, int c, float2 v) {
+            fz[(row0 + 2 * c) * N + n] = v.x;
+            fz[(row0 + 2 * c + 1) * N + n] = v.y;
+        },
+        lds, tw);
 }
 
 // --------------------------------------------------------- launchers ----
@@ -147,10 +225,10 @@ __global__ __launch_bounds__(NTC) void k_x_solve(float2* __restrict a, float2* _
 // means fewer elements, and registers, per thread: the fused x-solve holds the
 // spectrum between its three transforms and needs them.
 template <typename V, int CC, int NTC>
-void cols_launch(V* d, int axis, int dir, const V* tw, hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * (N / CC));
-    if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis);
-    else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis);
+void cols_launch(V* d, int axis, int dir, const V* tw, int pz, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * (pz / CC));
+    if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
+    else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
 }
 template <typename V, int RC>
 void rows_launch(V* d, int dir, const V* tw, hipStream_t s) {
@@ -186,34 +264,51 @@ void pass_f32(float2* d, int axis, int dir, const float2* tw, int variant, hipSt
     if (axis == 2) { rows_launch<float2, rc32()>(d, dir, tw, s); return; }
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, s);
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, N, s);
+    });
+}
+// Column pass over a half-spectrum grid N x N x HZ.
+void pass_half(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s) {
+    with_variant(variant, [&](auto vc) {
+        constexpr int v = decltype(vc)::value;
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, HZ, s);
     });
 }
 void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
     if (axis == 2) rows_launch<double2, rc64()>(d, dir, tw, s);
-    else cols_launch<double2, cc64(), 512>(d, axis, dir, tw, s);
+    else cols_launch<double2, cc64(), 512>(d, axis, dir, tw, N, s);
 }
 void fwd_z_density(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
                    hipStream_t s) {
     unsigned nb = (unsigned)((size_t)N * N / rc32());
     k_rows_density<rc32()><<<nb, NT, 0, s>>>(rho, total, out, tw);
 }
-void inv_z_real(const float2* in, float* out, const float2* tw, hipStream_t s) {
+void fwd_z_r2c(const long long* rho, const unsigned long long* total, float2* out, const float2* tw, hipStream_t s) {
+    constexpr int RC = 2 * rc32();
+    unsigned nb = (unsigned)((size_t)N * N / RC);
+    k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, total, out, tw);
+}
+void inv_z_c2r_xy(const float2* hx, const float2* hy, float2* fxy, const float2* tw, hipStream_t s) {
     unsigned nb = (unsigned)((size_t)N * N / rc32());
-    k_rows_real_out<rc32()><<<nb, NT, 0, s>>>(in, out, tw);
+    k_rows_c2r_xy<rc32()><<<nb, NT, 0, s>>>(hx, hy, fxy, tw);
+}
+void inv_z_c2r_pair(const float2* hz, float* fz, const float2* tw, hipStream_t s) {
+    constexpr int RC = 2 * rc32();
+    unsigned nb = (unsigned)((size_t)N * N / RC);
+    k_rows_c2r_pair<RC><<<nb, NT, 0, s>>>(hz, fz, tw);
 }
 template <int CC, int NTC>
-void x_solve_launch(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
-                    hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * (N / CC));
+void x_solve_r2c_launch(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
+                        const float* hs, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)N * (HZ / CC));
     const float gcoef = (float)(0.25 / ((double)N * N * N));
-    k_x_solve<CC, NTC><<<nb, NTC, 0, s>>>(a, p, twf, twr, sk, hs, gcoef);
+    k_x_solve_r2c<CC, NTC><<<nb, NTC, 0, s>>>(h1, h2, h3, twf, twr, sk, hs, gcoef);
 }
-void x_solve(float2* a, float2* p, const float2* twf, const float2* twr, const float* sk, const float* hs,
-             int variant, hipStream_t s) {
+void x_solve_r2c(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
+                 const float* hs, int variant, hipStream_t s) {
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        x_solve_launch<cc_of(v), nt_of(v)>(a, p, twf, twr, sk, hs, s);
+        x_solve_r2c_launch<cc_of(v), nt_of(v)>(h1, h2, h3, twf, twr, sk, hs, s);
     });
 }
 void twiddles(double* fwd, double* rev) {
@@ -228,7 +323,8 @@ namespace hgpm {
 #define HGPM_CAT2(a, b) a##b
 #define HGPM_CAT(a, b) HGPM_CAT2(a, b)
 extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
-    N, Plan<N>::S, NVAR, variant_mask(), twiddles, pass_f32, pass_f64, fwd_z_density, inv_z_real, x_solve,
+    N, Plan<N>::S, NVAR, variant_mask(), twiddles, pass_f32, pass_f64, fwd_z_density,
+    HZ, pass_half, fwd_z_r2c, x_solve_r2c, inv_z_c2r_xy, inv_z_c2r_pair,
 };
 } // namespace hgpm
 #endif

@@ -27,14 +27,21 @@ __global__ void k_zero_u64(unsigned long long* __restrict p, size_t n) {
     for (; i < n; i += st) p[i] = 0ull;
 }
 
-// Carry the integer part of an updated offset into the cell and wrap periodically.
+// Carry the integer part of an updated offset into the cell and wrap periodically:
+// c <- (c + floor(f) [+1 if f - floor(f) rounds to 1]) mod ng.  A move of less than one
+// box needs one compare-and-add; the integer division runs only beyond that.
 __device__ __forceinline__ void carry_wrap(int& c, float& f, int ng) {
     float fl = floorf(f);
     float r = f - fl;
-    c += (int)fl;
-    if (r >= 1.0f) { r = 0.0f; c += 1; }
-    c %= ng;
-    if (c < 0) c += ng;
+    int cc = c + (int)fl;
+    if (r >= 1.0f) { r = 0.0f; cc += 1; }
+    if (cc >= ng) cc -= ng;
+    else if (cc < 0) cc += ng;
+    if ((unsigned)cc >= (unsigned)ng) {
+        cc %= ng;
+        if (cc < 0) cc += ng;
+    }
+    c = cc;
     f = r;
 }
 
@@ -57,12 +64,18 @@ __global__ __launch_bounds__(BS) void k_drift(PosSplit* __restrict pos, const fl
     pos[i] = p;
 }
 
-// CIC weight of the corner (x,y,z) in FP32: ((dx*dy)*dz)*mass with dx = f or 1-f.
+// CIC weight of the corner (x,y,z) in FP32: ((dx*dy)*dz)*mass with dx = f or 1-f,
+// rounded to nearest-even in units of 2^-24.  SMALL (mass < 128, so the scaled weight
+// is below 2^31): round in FP32 and convert exactly through int32, the same value as
+// the int64 conversion at a fraction of its instructions.
+template <bool SMALL>
 __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, float mass) {
     float w = dx * dy;
     w = w * dz;
     w = w * mass;
-    return (long long)__float2ll_rn(w * 16777216.0f);
+    const float q = w * 16777216.0f;
+    if constexpr (SMALL) return (long long)(int)rintf(q);
+    else return (long long)__float2ll_rn(q);
 }
 
 // ----------------------------------------------------------------------------------
@@ -155,7 +168,7 @@ __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int
 // slab to the flush: ARENA stores every cell of the slab to arena[base + window index]
 // (window order x, y, z with extents e), otherwise the non-zero cells are added to the
 // grid with int64 atomics.  Returns this thread's share of the deposited mass.
-template <bool ARENA>
+template <bool ARENA, bool SMALL>
 __device__ __forceinline__ long long dep_window(unsigned long long* win, const DepRegs& R, const int lo[3],
                                                 const int e[3], const PosSplit& r0, int64_t p0, int64_t n, int ng,
                                                 float mass, unsigned long long* __restrict dst,
@@ -191,8 +204,8 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                         const int ly = by + y;
                         if (ly < 0 || ly >= ny) continue;
                         unsigned long long* row = &win[(lx * ny + ly) * e[2] + bz];
-                        const long long q0 = fixed_weight(gx[x], gy[y], gz[0], mass);
-                        const long long q1 = fixed_weight(gx[x], gy[y], gz[1], mass);
+                        const long long q0 = fixed_weight<SMALL>(gx[x], gy[y], gz[0], mass);
+                        const long long q1 = fixed_weight<SMALL>(gx[x], gy[y], gz[1], mass);
                         local += q0 + q1;
                         atomicAdd(row, (unsigned long long)q0);
                         atomicAdd(row + 1, (unsigned long long)q1);
@@ -207,10 +220,16 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                 const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
                 int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
                 const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+                // arena index of (ix, iy, iz), advanced with the walk: the y carry cancels,
+                // an x carry skips the rows of the window outside this slab
+                const int64_t plane = (int64_t)e[1] * ez;
+                int64_t ai = (int64_t)base + ((int64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz;
+                const int64_t astep = sz + (int64_t)sy * ez + (int64_t)sx * plane;
+                const int64_t askip = plane - (int64_t)ny * ez;
                 for (int i = tid; i < vol; i += DEP_NT) {
                     const unsigned long long v = win[i];
                     if (ARENA) {
-                        dst[base + ((uint64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz] = v;
+                        dst[ai] = v;
                     } else if (v) {
                         int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
                         gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
@@ -225,6 +244,7 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                     int cx = 0;
                     if (iy >= ny) { iy -= ny; cx = 1; }
                     ix += sx + cx;
+                    ai += astep + (cx ? askip : 0);
                 }
             }
             __syncthreads();
@@ -271,6 +291,7 @@ __device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3]
 // ctl[1] = deferred tiles, ctl[DEP_CTL_SUB + s] = cells reserved in sub-arena s.  The
 // arena is split into DEP_NSUB equal sub-arenas (tile % DEP_NSUB picks one) so that no
 // single counter takes every tile's reservation.
+template <bool SMALL>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __restrict pos, int64_t n, int ng,
                                                             float mass, unsigned long long* __restrict arena,
                                                             unsigned long long cap, uint4* __restrict slots,
@@ -346,7 +367,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
         }
         return;
     }
-    const long long local = dep_window<true>(win, R, lo, e, r0, p0, n, ng, mass, arena, s_base);
+    const long long local = dep_window<true, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, arena, s_base);
     dep_tile_total(local, s_tot, &tile_total[tile]);
 }
 
@@ -410,6 +431,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long lo
 }
 
 // Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
+template <bool SMALL>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __restrict pos, int64_t n, int ng,
                                                              float mass, unsigned long long* __restrict rho,
                                                              const unsigned long long* __restrict ctl,
@@ -426,7 +448,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
         DepRegs R;
         int lo[3], e[3];
         dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
-        const long long local = dep_window<false>(win, R, lo, e, r0, p0, n, ng, mass, rho, 0ull);
+        const long long local = dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, rho, 0ull);
         dep_tile_total(local, s_tot, &tile_total[tile]);
         __syncthreads();
     }
@@ -450,23 +472,27 @@ __global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long
 
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
 // summed over the 8 corners in (x,y,z) order, then optionally the drift.  A workgroup
-// takes GT_TILE consecutive particles (half a Lagrangian tile: an 8 x 16 x 16 block of
+// takes GT_TILE consecutive particles (part of a Lagrangian tile, a compact block of
 // the lattice).  The force cells under the bounding box of the particles' base cells
 // (plus the upper neighbours) are staged in LDS with coalesced loads, in slabs of at
 // most GMAX cells: a slab covers a range of base planes in x (and of base rows in y)
 // plus one overlapping plane (row), so each particle finds all 8 corners in the slab
 // that holds its base cell and is processed in exactly one slab.
-constexpr int GT_NT = 512;
-constexpr int GT_PPT = 4;
-constexpr int GT_TILE = GT_NT * GT_PPT;
-constexpr int GMAX = 6656; // 78 KB of (float2, float) per workgroup: two workgroups per CU
+// Workgroup shapes (threads x particles per thread, LDS budget in cells):
+//   variant 0: 512 x 4 = half a Lagrangian tile (8 x 16 x 16), 6656 cells (78 KB, 2 per CU)
+//   variant 1: 256 x 4 = a quarter tile (4 x 16 x 16), 4160 cells (49 KB, 3 per CU)
+// The budget must hold two base rows plus their neighbours of any window,
+// 4 x (ng + 1) cells for ng <= 1024.
 constexpr int GT_STAGE = 8; // staged cells per thread per load round
-static_assert(DEP_TILE % GT_TILE == 0, "tile");
 
+template <int GT_NT, int GT_PPT, int GMAX>
 __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float2* __restrict fxy,
                                                        const float* __restrict fz, double deltaT, double fscal,
                                                        int do_drift, float pf) {
+    constexpr int GT_TILE = GT_NT * GT_PPT;
+    static_assert(DEP_TILE % GT_TILE == 0, "tile");
+    static_assert(GMAX >= 4 * 1025, "two rows of the widest window");
     __shared__ float2 sxy[GMAX];
     __shared__ float sz[GMAX];
     __shared__ int red[6][GT_NT / 64];
@@ -524,15 +550,27 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                 const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % nys, sx = syx / nys;
                 int iz = tid % ez, iy = (tid / ez) % nys, ix = (tid / ez) / nys;
                 const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+                // a slab that does not cross the periodic boundary (uniform per workgroup)
+                // advances the grid index with the walk instead of wrapping each cell
+                const bool inside = bx0 >= 0 && bx0 + nxs <= ng && by0 >= 0 && by0 + nys <= ng && bz0 >= 0 &&
+                                    bz0 + ez <= ng;
+                const unsigned ung = (unsigned)ng, ung2 = ung * ung;
+                unsigned gi = ((unsigned)(bx0 + ix) * ung + (unsigned)(by0 + iy)) * ung + (unsigned)(bz0 + iz);
+                const unsigned gstep = (unsigned)sz_ + (unsigned)sy * ung + (unsigned)sx * ung2;
+                const unsigned gcy = ung - (unsigned)ez, gcx = ung2 - (unsigned)nys * ung;
                 for (int i0 = tid; i0 < vol; i0 += GT_STAGE * GT_NT) {
                     unsigned g[GT_STAGE];
 #pragma unroll
                     for (int u = 0; u < GT_STAGE; u++) {
-                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
-                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                        g[u] = ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng + (unsigned)gz;
+                        if (inside) {
+                            g[u] = gi;
+                        } else {
+                            int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                            gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+                            g[u] = ((unsigned)gx * ung + (unsigned)gy) * ung + (unsigned)gz;
+                        }
                         iz += sz_;
                         int cy = 0;
                         if (iz >= ez) { iz -= ez; cy = 1; }
@@ -540,6 +578,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                         int cx = 0;
                         if (iy >= nys) { iy -= nys; cx = 1; }
                         ix += sx + cx;
+                        gi += gstep + (cy ? gcy : 0u) + (cx ? gcx : 0u);
                     }
                     float2 a[GT_STAGE];
                     float c[GT_STAGE];
@@ -717,8 +756,14 @@ void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass,
                            hipStream_t s) {
     if (!n) return;
     const unsigned nt = (unsigned)(n / DEP_TILE);
-    k_deposit_windows<<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap, (uint4*)b.slots,
-                                            b.count, b.nslots, b.deferred, b.ctl, b.tile_total);
+    if (mass < 128.0f)
+        k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
+                                                      (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
+                                                      b.tile_total);
+    else
+        k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
+                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
+                                                       b.tile_total);
 }
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
@@ -730,15 +775,26 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass
     if (!n) return;
     const int64_t nt = n / DEP_TILE;
     const unsigned grid = (unsigned)(nt < 512 ? nt : 512);
-    k_deposit_deferred<<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho, b.ctl,
-                                               b.deferred, b.tile_total);
+    if (mass < 128.0f)
+        k_deposit_deferred<true><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho,
+                                                         b.ctl, b.deferred, b.tile_total);
+    else
+        k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass,
+                                                          (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total);
     k_deposit_total<<<1, 1024, 0, s>>>(b.tile_total, nt, b.ctl);
 }
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, hipStream_t s) {
+                        double deltaT, double fscal, int do_drift, float pf, int variant, hipStream_t s) {
     if (!n) return;
-    unsigned nb = (unsigned)((n + GT_TILE - 1) / GT_TILE);
-    k_gather_slab<<<nb, GT_NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf);
+    if (variant == 1) {
+        const unsigned nb = (unsigned)(n / (256 * 4));
+        k_gather_slab<256, 4, 4160><<<nb, 256, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
+                                                       do_drift, pf);
+    } else {
+        const unsigned nb = (unsigned)(n / (512 * 4));
+        k_gather_slab<512, 4, 6656><<<nb, 512, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
+                                                       do_drift, pf);
+    }
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {

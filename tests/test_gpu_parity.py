@@ -204,12 +204,17 @@ def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
         m.solve_gradient()
         F = m.get_force()
         dT, fs, pf = 0.0123, 1.87, 0.75
-        # kick alone
-        m.update_velocities(dT, fs)
-        c1, o1, v1 = m.get_particles_split()
         vref = orc.kick_split(ng, cell, off, vel, F, dT, fs)
-        assert np.array_equal(v1, vref)
-        assert np.array_equal(c1, cell) and np.array_equal(o1, off)
+        # kick alone, with every gather workgroup shape
+        for gt in (1, 0):
+            m.set_tunable("gather_tile", gt)
+            m.set_particles_split(cell, off, vel)
+            m.cic()
+            m.solve_gradient()
+            m.update_velocities(dT, fs)
+            c1, o1, v1 = m.get_particles_split()
+            assert np.array_equal(v1, vref), gt
+            assert np.array_equal(c1, cell) and np.array_equal(o1, off)
         # fused kick + drift from the same start
         m.set_particles_split(cell, off, vel)
         m.cic()
