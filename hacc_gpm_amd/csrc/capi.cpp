@@ -225,6 +225,9 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.deferred = c->deferred;
     b.ctl = c->ctl;
     b.tile_total = c->tile_total;
+    // about 256 tiles per sub-arena, at most HGPM_DEP_NSUB sub-arenas
+    const int64_t nt = c->N / (16 * 16 * 16);
+    b.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(HGPM_DEP_NSUB, nt / 256));
     return b;
 }
 

@@ -69,6 +69,7 @@ struct DepositBuffers {
     unsigned* deferred;
     unsigned long long* ctl;
     unsigned long long* tile_total;
+    int nsub; // sub-arenas in use, 1..HGPM_DEP_NSUB
 };
 void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);

@@ -137,7 +137,7 @@ __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
         const int64_t j = p0 + tid + k * DEP_NT;
-        const PosSplit p = j < n ? pos[j] : r0;
+        const PosSplit p = pos[j]; // tiles are full: n is a multiple of DEP_TILE
         R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
         R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
         R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
@@ -190,7 +190,6 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                 // compiler from hoisting all 64 of them out of the slab loop (register spill)
                 float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
                 asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
-                if (p0 + tid + k * DEP_NT >= n) continue;
                 const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0, bz = R.rz[k] - lo[2];
                 const float gx[2] = {1.0f - ox, ox};
                 const float gy[2] = {1.0f - oy, oy};
@@ -289,8 +288,9 @@ __device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3]
 }
 
 // ctl[1] = deferred tiles, ctl[DEP_CTL_SUB + s] = cells reserved in sub-arena s.  The
-// arena is split into DEP_NSUB equal sub-arenas (tile % DEP_NSUB picks one) so that no
-// single counter takes every tile's reservation.
+// arena is split into nsub <= DEP_NSUB equal sub-arenas (tile % nsub picks one) so that
+// no single counter takes every tile's reservation; a few hundred tiles per sub-arena
+// let their window sizes average out.
 template <bool SMALL>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __restrict pos, int64_t n, int ng,
                                                             float mass, unsigned long long* __restrict arena,
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
                                                             unsigned* __restrict count, int nslots,
                                                             unsigned* __restrict deferred,
                                                             unsigned long long* __restrict ctl,
-                                                            unsigned long long* __restrict tile_total) {
+                                                            unsigned long long* __restrict tile_total, int nsub) {
     __shared__ unsigned long long win[WMAX];
     __shared__ int red[6][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
@@ -325,8 +325,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     if (tid == 0) {
         int st = 1;
         if (nch <= DEP_MAXCH) {
-            const unsigned long long sub_cap = cap / DEP_NSUB;
-            const int sub = tile % DEP_NSUB;
+            const unsigned long long sub_cap = cap / nsub;
+            const int sub = tile % nsub;
             const unsigned long long b = atomicAdd(&ctl[DEP_CTL_SUB + sub], V);
             s_base = (unsigned long long)sub * sub_cap + b;
             st = (b + V <= sub_cap) ? 0 : 1;
@@ -414,8 +414,12 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long lo
                 const int r = i - lx * myz;
                 const int ly = (int)(((float)r + 0.5f) * rmz);
                 const int lz = r - ly * mz;
-                at[u] = i < vol ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
-                v[u] = i < vol ? arena[off + ((uint64_t)lx * e1 + ly) * e2 + lz] : 0ull;
+                // loads are unconditional (cell 0 of the chunk past its end) so that the
+                // eight of a round are in flight together
+                const bool in = i < vol;
+                at[u] = in ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
+                const unsigned long long a = arena[off + (in ? ((uint64_t)lx * e1 + ly) * e2 + lz : 0ull)];
+                v[u] = in ? a : 0ull;
             }
 #pragma unroll
             for (int u = 0; u < OWN_U; u++)
@@ -505,10 +509,10 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
 #pragma unroll
     for (int k = 0; k < GT_PPT; k++) {
         const int64_t j = p0 + tid + k * GT_NT;
-        P[k] = j < n ? pos[j] : r0;
-        V[k][0] = j < n ? vel[3 * j] : 0.0f;
-        V[k][1] = j < n ? vel[3 * j + 1] : 0.0f;
-        V[k][2] = j < n ? vel[3 * j + 2] : 0.0f;
+        P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
+        V[k][0] = vel[3 * j];
+        V[k][1] = vel[3 * j + 1];
+        V[k][2] = vel[3 * j + 2];
         const int ax = wrap_rel(P[k].cx - r0.cx, ng), ay = wrap_rel(P[k].cy - r0.cy, ng),
                   az = wrap_rel(P[k].cz - r0.cz, ng);
         mn[0] = min(mn[0], ax); mx[0] = max(mx[0], ax);
@@ -555,6 +559,10 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                 const bool inside = bx0 >= 0 && bx0 + nxs <= ng && by0 >= 0 && by0 + nys <= ng && bz0 >= 0 &&
                                     bz0 + ez <= ng;
                 const unsigned ung = (unsigned)ng, ung2 = ung * ung;
+                const int wx0 = bx0 < 0 ? bx0 + ng : (bx0 >= ng ? bx0 - ng : bx0);
+                const int wy0 = by0 < 0 ? by0 + ng : (by0 >= ng ? by0 - ng : by0);
+                const int wz0 = bz0 < 0 ? bz0 + ng : (bz0 >= ng ? bz0 - ng : bz0);
+                const unsigned g0 = ((unsigned)wx0 * ung + (unsigned)wy0) * ung + (unsigned)wz0;
                 unsigned gi = ((unsigned)(bx0 + ix) * ung + (unsigned)(by0 + iy)) * ung + (unsigned)(bz0 + iz);
                 const unsigned gstep = (unsigned)sz_ + (unsigned)sy * ung + (unsigned)sx * ung2;
                 const unsigned gcy = ung - (unsigned)ez, gcx = ung2 - (unsigned)nys * ung;
@@ -584,10 +592,11 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                     float c[GT_STAGE];
 #pragma unroll
                     for (int u = 0; u < GT_STAGE; u++) {
-                        if (i0 + u * GT_NT < vol) {
-                            a[u] = fxy[g[u]];
-                            c[u] = fz[g[u]];
-                        }
+                        // unconditional loads (the slab's first cell past its end) keep all
+                        // GT_STAGE of them in flight together
+                        const unsigned gg = i0 + u * GT_NT < vol ? g[u] : g0;
+                        a[u] = fxy[gg];
+                        c[u] = fz[gg];
                     }
 #pragma unroll
                     for (int u = 0; u < GT_STAGE; u++) {
@@ -605,7 +614,6 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                 // resident workgroups per CU allow
                 __builtin_amdgcn_sched_barrier(0);
                 const int64_t j = p0 + tid + k * GT_NT;
-                if (j >= n) continue;
                 // an opaque copy keeps the slab-invariant weights and indices of all
                 // GT_PPT particles from being hoisted out of the slab loops (registers)
                 PosSplit p = P[k];
@@ -759,11 +767,11 @@ void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass,
     if (mass < 128.0f)
         k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                      b.tile_total);
+                                                      b.tile_total, b.nsub);
     else
         k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                        (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                       b.tile_total);
+                                                       b.tile_total, b.nsub);
 }
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
