@@ -55,6 +55,8 @@ struct hgpm_ctx {
     unsigned* dcount = nullptr;
     unsigned* deferred = nullptr;
     unsigned long long* tile_total = nullptr;
+    void* boxes = nullptr;   // half-tile cell boxes of the last windows pass
+    bool boxes_valid = false; // ... and the positions have not changed since
     int nslots = HGPM_DEP_SLOTS;
     float2 *twf = nullptr, *twr = nullptr, *twf_ic = nullptr;
     float *sk = nullptr, *hs = nullptr;
@@ -225,6 +227,7 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.deferred = c->deferred;
     b.ctl = c->ctl;
     b.tile_total = c->tile_total;
+    b.boxes = c->boxes;
     // about 256 tiles per sub-arena, at most HGPM_DEP_NSUB sub-arenas
     const int64_t nt = c->N / (16 * 16 * 16);
     b.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(HGPM_DEP_NSUB, nt / 256));
@@ -240,6 +243,7 @@ int f32_deposit(hgpm_ctx* c) {
     {
         Timed t1(c, "deposit_windows");
         launch_deposit_windows(c->pos, c->N, c->ng, ref_mass(c), b, c->st);
+        c->boxes_valid = true;
     }
     {
         Timed t2(c, "deposit_owner");
@@ -254,7 +258,8 @@ int f32_deposit(hgpm_ctx* c) {
 }
 
 // Poisson solve and gradient on half spectra: density (B, int64) -> rho^ (A, pitch hz)
-// -> Fx^, Fy^, Fz^ (h2, h3, A) -> (Fx, Fy) float2 in B and Fz in fz.
+// -> Fx^, Fy^ (h2, h3) and U = i g rho^ (A, Fz^ = sin k_z U) -> (Fx, Fy) float2 in B and
+// Fz in fz.
 int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {
@@ -281,7 +286,7 @@ int f32_solve(hgpm_ctx* c) {
     }
     {
         Timed t(c, "fft_inv_z_fz");
-        f->inv_z_c2r_pair(c->A, c->fz, c->twf, c->st);
+        f->inv_z_c2r_pair(c->A, c->sk, c->fz, c->twf, c->st);
     }
     LAUNCHCHK(c);
     return HGPM_OK;
@@ -436,7 +441,8 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
                 (rc = dalloc(c, &c->dcount, sizeof(unsigned) * nblk, "deposit counts")) ||
                 (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||
-                (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")))
+                (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
+                (rc = dalloc(c, (char**)&c->boxes, 64 * ntiles, "gather boxes")))
                 return bail(rc);
             if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess) {
                 c->err = "hipMemset failed";
@@ -503,7 +509,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->twf, c->twr, c->sk, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->sk, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -533,6 +539,7 @@ int hgpm_synchronize(hgpm_ctx* c) {
 int hgpm_set_particles(hgpm_ctx* c, const float* pos, const float* vel) {
     if (!c || !pos || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles: null argument");
     const size_t n3 = 3 * (size_t)c->N;
+    c->boxes_valid = false;
     if (c->prec == HGPM_FP32) {
         float* tmp = nullptr;
         int rc = dalloc(c, &tmp, sizeof(float) * 2 * n3, "staging");
@@ -593,6 +600,7 @@ int hgpm_get_particles(hgpm_ctx* c, float* pos, float* vel) {
 int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off, const float* vel) {
     if (!c || !cell || !off || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles_split: null argument");
     if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "split layout needs an FP32 context");
+    c->boxes_valid = false;
     const size_t n = (size_t)c->N;
     std::vector<PosSplitH> h(n);
     for (size_t i = 0; i < n; i++) {
@@ -690,6 +698,7 @@ int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_
     // the interpolation reads bin (int)(k/k_delta)+1 for |k| up to sqrt(3) pi np/rl
     double kmax_grid = sqrt(3.0) * (np / 2) * 2 * HGPM_PI_D / c->p.rl;
     if ((int)(kmax_grid / ipk_k_delta) + 1 >= ipk_bins) return fail(c, HGPM_ERR_ARG, "ipk table too short for this grid");
+    c->boxes_valid = false;
     double* d_ipk = nullptr;
     int rc = dalloc(c, &d_ipk, sizeof(double) * ipk_bins, "ipk");
     if (rc) return rc;
@@ -736,6 +745,7 @@ int hgpm_update_positions(hgpm_ctx* c, double prefactor) {
     int rc = need_particles(c);
     if (rc) return rc;
     if (c->prec == HGPM_FP32) {
+        c->boxes_valid = false;
         Timed t(c, "drift");
         launch_drift(c->pos, c->vel, c->N, c->ng, (float)prefactor, c->st);
     } else {
@@ -865,7 +875,8 @@ int hgpm_update_velocities(hgpm_ctx* c, double deltaT, double fscal) {
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick");
-        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather, c->st);
+        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather,
+                           c->boxes_valid ? c->boxes : nullptr, c->st);
     } else {
         Timed t(c, "f64_icic");
         launch_f64_icic(c->pos64, c->vel64, c->grad64, c->N, c->ng, deltaT, fscal, c->st);
@@ -882,7 +893,8 @@ int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) 
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick_drift");
         launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor, c->var_gather,
-                           c->st);
+                           c->boxes_valid ? c->boxes : nullptr, c->st);
+        c->boxes_valid = false;
         LAUNCHCHK(c);
         return HGPM_OK;
     }
@@ -966,6 +978,7 @@ int hgpm_power_spectrum(hgpm_ctx* c, int fold, double* k, double* pk, int64_t* c
             c->pos = tmp;
             rc = f32_deposit(c);
             c->pos = keep;
+            c->boxes_valid = false; // the boxes are those of the folded copy
             HIPCHK(c, hipStreamSynchronize(c->st));
             (void)hipFree(tmp);
             if (rc) return rc;

@@ -26,7 +26,8 @@ struct FftOps {
     void (*x_solve_r2c)(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr,
                         const float* sk, const float* hs, int variant, hipStream_t s);
     void (*inv_z_c2r_xy)(const float2* hx, const float2* hy, float2* fxy, const float2* tw, hipStream_t s);
-    void (*inv_z_c2r_pair)(const float2* hz, float* fz, const float2* tw, hipStream_t s);
+    // Fz from U = i g rho^ (half spectra after the inverse x, y passes): Fz^ = sk[k_z] U
+    void (*inv_z_c2r_pair)(const float2* hz, const float* sk, float* fz, const float2* tw, hipStream_t s);
 };
 
 #define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
@@ -70,6 +71,7 @@ struct DepositBuffers {
     unsigned long long* ctl;
     unsigned long long* tile_total;
     int nsub; // sub-arenas in use, 1..HGPM_DEP_NSUB
+    void* boxes; // 32 B per half tile: the cell box the gather stages (written by the windows pass)
 };
 void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
@@ -77,8 +79,11 @@ void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipSt
 void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s);
 // variant 0: 512-thread workgroups on half tiles; 1: 256-thread workgroups on quarter tiles
+// boxes: the half-tile boxes of the last deposit when the positions have not changed
+// since, else nullptr (the gather reduces them itself)
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, int variant, hipStream_t s);
+                        double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
+                        hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
                               int ng, hipStream_t s);

@@ -14,7 +14,9 @@
 //   r2c   : pairs of real rows a, b as one complex transform of a + i b, split into
 //           the half spectra of a and b;
 //   x_solve_r2c: forward x transform, then the inverse x transforms of
-//           Fx^ = i g sin k_x rho^, Fy^ = i g sin k_y rho^, Fz^ = i g sin k_z rho^;
+//           Fx^ = i g sin k_x rho^ and of U = i g rho^: sin k_y and sin k_z are constant
+//           along an x column, so Fy^ = sin k_y U is written at once and sin k_z is
+//           applied when the z pass reads U (two inverse transforms, not three);
 //   c2r   : (Fx, Fy) as one complex transform of the Hermitian extension of
 //           Fx^ + i Fy^; Fz rows in pairs the same way.
 #include "fft.h"
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __rest
 // i g sin(k_d) rho^ with G = (0.5/N^3)/(sum_d cos k_d - 3) = -g,
 // g = (0.25/N^3)/sum_d sin^2(k_d/2), G(0) = 0 (kspace_solve_gradient,
 // src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens, src/pm_solver/cuda/calcGreens.hpp:11-26):
-// Fx^ -> h2, Fy^ -> h3, Fz^ -> h1, without the spectrum leaving registers.  sk[m] =
+// Fx^ -> h2; U = i g rho^ -> h1 and Fy^ = sin k_y U -> h3 (Fz^ = sin k_z U is formed by
+// k_rows_c2r_pair), without the spectrum leaving registers.  sk[m] =
 // sin(2 pi m_s/N) with the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274);
 // hs[m] = sin^2(pi m/N).
 template <int CC, int NTC>
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, floa
     float2 R[LS::NB][LS::R];
     const int c0 = threadIdx.x % CC;
     const int z = zc * CC + c0; // < HZ <= N
-    const float sy = sk[y], sz = sk[z];
+    const float sy = sk[y];
     float G[LS::NB][LS::R], SX[LS::NB][LS::R];
 #pragma unroll
     for (int b = 0; b < LS::NB; b++)
@@ -164,16 +167,14 @@ __global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, floa
         [&](int, int, int n, int c, float2 v) { h2[base + n * NP + c] = v; }, lds, twr);
     fft_tile<float2, N, CC, NTC, +1, true, false>(
         [&](int b, int r, int, int) {
-            const float f = G[b][r] * sy;
+            const float f = G[b][r];
             return make_float2(-f * R[b][r].y, f * R[b][r].x);
         },
-        [&](int, int, int n, int c, float2 v) { h3[base + n * NP + c] = v; }, lds, twr);
-    fft_tile<float2, N, CC, NTC, +1, true, false>(
-        [&](int b, int r, int, int) {
-            const float f = G[b][r] * sz;
-            return make_float2(-f * R[b][r].y, f * R[b][r].x);
+        [&](int, int, int n, int c, float2 v) {
+            h1[base + n * NP + c] = v;
+            h3[base + n * NP + c] = make_float2(sy * v.x, sy * v.y);
         },
-        [&](int, int, int n, int c, float2 v) { h1[base + n * NP + c] = v; }, lds, twr);
+        lds, twr);
 }
 
 // Element n of the Hermitian extension of a half-spectrum row: h[n] for n <= N/2,
@@ -199,16 +200,20 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_xy(const float2* __restrict hx,
         [&](int, int, int n, int c, float2 v) { fxy[(row0 + c) * N + n] = v; }, lds, tw);
 }
 
-// Inverse z transform of Fz^ half spectra, rows in pairs: C = Fz^_a + i Fz^_b.
+// Inverse z transform of Fz^ = sin k_z U from the half spectra U, rows in pairs:
+// C = Fz^_a + i Fz^_b over the Hermitian extension (sin k_z is real).
 template <int RC>
-__global__ __launch_bounds__(NT) void k_rows_c2r_pair(const float2* __restrict hz, float* __restrict fz,
-                                                      const float2* __restrict tw) {
+__global__ __launch_bounds__(NT) void k_rows_c2r_pair(const float2* __restrict hz, const float* __restrict sk,
+                                                      float* __restrict fz, const float2* __restrict tw) {
     constexpr int PC = RC / 2;
     __shared__ float2 lds[lds_elems<N, PC>()];
     const size_t row0 = (size_t)blockIdx.x * RC;
     fft_tile<float2, N, PC, NT, +1, false, true>(
         [&](int, int, int n, int c) {
-            const float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
+            const float s = sk[n <= N / 2 ? n : N - n];
+            float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
+            a.x *= s; a.y *= s;
+            b.x *= s; b.y *= s;
             return make_float2(a.x - b.y, a.y + b.x);
         },
         [&](int, int, int n, int c, float2 v) {
@@ -292,10 +297,10 @@ void inv_z_c2r_xy(const float2* hx, const float2* hy, float2* fxy, const float2*
     unsigned nb = (unsigned)((size_t)N * N / rc32());
     k_rows_c2r_xy<rc32()><<<nb, NT, 0, s>>>(hx, hy, fxy, tw);
 }
-void inv_z_c2r_pair(const float2* hz, float* fz, const float2* tw, hipStream_t s) {
+void inv_z_c2r_pair(const float2* hz, const float* sk, float* fz, const float2* tw, hipStream_t s) {
     constexpr int RC = 2 * rc32();
     unsigned nb = (unsigned)((size_t)N * N / RC);
-    k_rows_c2r_pair<RC><<<nb, NT, 0, s>>>(hz, fz, tw);
+    k_rows_c2r_pair<RC><<<nb, NT, 0, s>>>(hz, sk, fz, tw);
 }
 template <int CC, int NTC>
 void x_solve_r2c_launch(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,

@@ -128,39 +128,55 @@ struct DepRegs {
 };
 
 // Loads the tile starting at p0 and reduces the box of its cells: lo = minimum
-// relative cell, e = extent + 1 (the upper CIC neighbour).
+// relative cell, e = extent + 1 (the upper CIC neighbour).  The two halves of the tile
+// (particles k < DEP_PPT/2 and k >= DEP_PPT/2 of every thread: the gather's tiles) get
+// their own boxes too: hb[h] = {lo[3], base cells per axis[3]}.
 __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int64_t p0, int64_t n, int ng,
                                              const PosSplit& r0, DepRegs& R, int (*red)[DEP_NT / 64], int lo[3],
-                                             int e[3]) {
+                                             int e[3], int (*hb)[6] = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+    int mn[2][3], mx[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) { mn[h][d] = INT_MAX; mx[h][d] = INT_MIN; }
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
+        const int h = k < DEP_PPT / 2 ? 0 : 1;
         const int64_t j = p0 + tid + k * DEP_NT;
         const PosSplit p = pos[j]; // tiles are full: n is a multiple of DEP_TILE
         R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
         R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
         R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
         R.fx[k] = p.fx; R.fy[k] = p.fy; R.fz[k] = p.fz;
-        mn[0] = min(mn[0], R.rx[k]); mx[0] = max(mx[0], R.rx[k]);
-        mn[1] = min(mn[1], R.ry[k]); mx[1] = max(mx[1], R.ry[k]);
-        mn[2] = min(mn[2], R.rz[k]); mx[2] = max(mx[2], R.rz[k]);
+        mn[h][0] = min(mn[h][0], R.rx[k]); mx[h][0] = max(mx[h][0], R.rx[k]);
+        mn[h][1] = min(mn[h][1], R.ry[k]); mx[h][1] = max(mx[h][1], R.ry[k]);
+        mn[h][2] = min(mn[h][2], R.rz[k]); mx[h][2] = max(mx[h][2], R.rz[k]);
     }
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        for (int off = 32; off > 0; off >>= 1) {
-            mn[d] = min(mn[d], __shfl_xor(mn[d], off, 64));
-            mx[d] = max(mx[d], __shfl_xor(mx[d], off, 64));
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            for (int off = 32; off > 0; off >>= 1) {
+                mn[h][d] = min(mn[h][d], __shfl_xor(mn[h][d], off, 64));
+                mx[h][d] = max(mx[h][d], __shfl_xor(mx[h][d], off, 64));
+            }
+            if (lane == 0) { red[6 * h + d][wv] = mn[h][d]; red[6 * h + 3 + d][wv] = mx[h][d]; }
         }
-        if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
-    }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        int a = red[d][0], b = red[3 + d][0];
-        for (int w = 1; w < DEP_NT / 64; w++) { a = min(a, red[d][w]); b = max(b, red[3 + d][w]); }
-        lo[d] = a;
-        e[d] = b - a + 2;
+        int a0 = red[d][0], b0 = red[3 + d][0], a1 = red[6 + d][0], b1 = red[9 + d][0];
+        for (int w = 1; w < DEP_NT / 64; w++) {
+            a0 = min(a0, red[d][w]); b0 = max(b0, red[3 + d][w]);
+            a1 = min(a1, red[6 + d][w]); b1 = max(b1, red[9 + d][w]);
+        }
+        lo[d] = min(a0, a1);
+        e[d] = max(b0, b1) - lo[d] + 2;
+        if (hb) {
+            hb[0][d] = a0; hb[0][3 + d] = b0 - a0 + 1;
+            hb[1][d] = a1; hb[1][3 + d] = b1 - a1 + 1;
+        }
     }
 }
 
@@ -298,9 +314,10 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
                                                             unsigned* __restrict count, int nslots,
                                                             unsigned* __restrict deferred,
                                                             unsigned long long* __restrict ctl,
-                                                            unsigned long long* __restrict tile_total, int nsub) {
+                                                            unsigned long long* __restrict tile_total, int nsub,
+                                                            int4* __restrict boxes) {
     __shared__ unsigned long long win[WMAX];
-    __shared__ int red[6][DEP_NT / 64];
+    __shared__ int red[12][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
     __shared__ unsigned long long s_base;
     __shared__ int s_state; // 0 arena route, 1 deferred (nothing reserved), 2 deferred (slots voided)
@@ -310,8 +327,15 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     const int64_t p0 = (int64_t)tile * DEP_TILE;
     const PosSplit r0 = pos[p0];
     DepRegs R;
-    int lo[3], e[3];
-    dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
+    int lo[3], e[3], hb[2][6];
+    dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
+    if (tid == 0) {
+        // the gather of these positions stages its force window from these boxes
+        boxes[4 * tile + 0] = make_int4(hb[0][0], hb[0][1], hb[0][2], hb[0][3]);
+        boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
+        boxes[4 * tile + 2] = make_int4(hb[1][0], hb[1][1], hb[1][2], hb[1][3]);
+        boxes[4 * tile + 3] = make_int4(hb[1][4], hb[1][5], 0, 0);
+    }
     const int nb = ng / OB;
     const int A[3] = {r0.cx + lo[0], r0.cy + lo[1], r0.cz + lo[2]};
     int c0[3], nc[3];
@@ -442,7 +466,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
                                                              const unsigned* __restrict deferred,
                                                              unsigned long long* __restrict tile_total) {
     __shared__ unsigned long long win[WMAX];
-    __shared__ int red[6][DEP_NT / 64];
+    __shared__ int red[12][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
     const unsigned long long nd = ctl[1];
     for (unsigned long long q = blockIdx.x; q < nd; q += gridDim.x) {
@@ -489,52 +513,70 @@ __global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long
 // 4 x (ng + 1) cells for ng <= 1024.
 constexpr int GT_STAGE = 8; // staged cells per thread per load round
 
-template <int GT_NT, int GT_PPT, int GMAX>
+template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
 __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float2* __restrict fxy,
                                                        const float* __restrict fz, double deltaT, double fscal,
-                                                       int do_drift, float pf) {
+                                                       int do_drift, float pf, const int4* __restrict boxes) {
     constexpr int GT_TILE = GT_NT * GT_PPT;
-    static_assert(DEP_TILE % GT_TILE == 0, "tile");
+    static_assert(DEP_TILE % GT_TILE == 0 && GT_TILE <= DEP_TILE / 2, "tile");
     static_assert(GMAX >= 4 * 1025, "two rows of the widest window");
     __shared__ float2 sxy[GMAX];
     __shared__ float sz[GMAX];
     __shared__ int red[6][GT_NT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
-    const PosSplit r0 = pos[p0];
+    // BOXED: the deposit of these positions stored the box of each half deposit tile,
+    // relative to the deposit tile's first particle; no reduction, and the staging
+    // loads can start while the particle loads are in flight
+    const PosSplit r0 = pos[BOXED ? p0 / DEP_TILE * DEP_TILE : p0];
     PosSplit P[GT_PPT];
     float V[GT_PPT][3];
-    int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
-#pragma unroll
-    for (int k = 0; k < GT_PPT; k++) {
-        const int64_t j = p0 + tid + k * GT_NT;
-        P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
-        V[k][0] = vel[3 * j];
-        V[k][1] = vel[3 * j + 1];
-        V[k][2] = vel[3 * j + 2];
-        const int ax = wrap_rel(P[k].cx - r0.cx, ng), ay = wrap_rel(P[k].cy - r0.cy, ng),
-                  az = wrap_rel(P[k].cz - r0.cz, ng);
-        mn[0] = min(mn[0], ax); mx[0] = max(mx[0], ax);
-        mn[1] = min(mn[1], ay); mx[1] = max(mx[1], ay);
-        mn[2] = min(mn[2], az); mx[2] = max(mx[2], az);
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        for (int off = 32; off > 0; off >>= 1) {
-            mn[d] = min(mn[d], __shfl_xor(mn[d], off, 64));
-            mx[d] = max(mx[d], __shfl_xor(mx[d], off, 64));
-        }
-        if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
-    }
-    __syncthreads();
     int lo[3], b[3];
+    if constexpr (BOXED) {
+        const int64_t hbi = p0 / (DEP_TILE / 2);
+        const int4 q0 = boxes[2 * hbi], q1 = boxes[2 * hbi + 1];
+        lo[0] = q0.x; lo[1] = q0.y; lo[2] = q0.z;
+        b[0] = q0.w; b[1] = q1.x; b[2] = q1.y;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        int a = red[d][0], c = red[3 + d][0];
-        for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); c = max(c, red[3 + d][w]); }
-        lo[d] = a;
-        b[d] = c - a + 1; // base cells per axis; the staged extent adds the upper neighbour
+        for (int k = 0; k < GT_PPT; k++) {
+            const int64_t j = p0 + tid + k * GT_NT;
+            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
+            V[k][0] = vel[3 * j];
+            V[k][1] = vel[3 * j + 1];
+            V[k][2] = vel[3 * j + 2];
+        }
+    } else {
+        int mn[3] = {INT_MAX, INT_MAX, INT_MAX}, mx[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int k = 0; k < GT_PPT; k++) {
+            const int64_t j = p0 + tid + k * GT_NT;
+            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
+            V[k][0] = vel[3 * j];
+            V[k][1] = vel[3 * j + 1];
+            V[k][2] = vel[3 * j + 2];
+            const int ax = wrap_rel(P[k].cx - r0.cx, ng), ay = wrap_rel(P[k].cy - r0.cy, ng),
+                      az = wrap_rel(P[k].cz - r0.cz, ng);
+            mn[0] = min(mn[0], ax); mx[0] = max(mx[0], ax);
+            mn[1] = min(mn[1], ay); mx[1] = max(mx[1], ay);
+            mn[2] = min(mn[2], az); mx[2] = max(mx[2], az);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            for (int off = 32; off > 0; off >>= 1) {
+                mn[d] = min(mn[d], __shfl_xor(mn[d], off, 64));
+                mx[d] = max(mx[d], __shfl_xor(mx[d], off, 64));
+            }
+            if (lane == 0) { red[d][wv] = mn[d]; red[3 + d][wv] = mx[d]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            int a = red[d][0], c = red[3 + d][0];
+            for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); c = max(c, red[3 + d][w]); }
+            lo[d] = a;
+            b[d] = c - a + 1; // base cells per axis; the staged extent adds the upper neighbour
+        }
     }
     const int ez = b[2] + 1;
     // slab shape: ty base rows (+1 staged) with (ty+1)*ez <= GMAX/2, then tx base planes
@@ -767,11 +809,11 @@ void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass,
     if (mass < 128.0f)
         k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                      b.tile_total, b.nsub);
+                                                      b.tile_total, b.nsub, (int4*)b.boxes);
     else
         k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                        (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                       b.tile_total, b.nsub);
+                                                       b.tile_total, b.nsub, (int4*)b.boxes);
 }
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
@@ -791,18 +833,23 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass
                                                           (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total);
     k_deposit_total<<<1, 1024, 0, s>>>(b.tile_total, nt, b.ctl);
 }
+template <int NT, int GM>
+void gather_launch(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz, double deltaT,
+                   double fscal, int do_drift, float pf, const void* boxes, hipStream_t s) {
+    const unsigned nb = (unsigned)(n / (NT * 4));
+    if (boxes)
+        k_gather_slab<NT, 4, GM, true><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
+                                                         do_drift, pf, (const int4*)boxes);
+    else
+        k_gather_slab<NT, 4, GM, false><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
+                                                          do_drift, pf, nullptr);
+}
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, int variant, hipStream_t s) {
+                        double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
+                        hipStream_t s) {
     if (!n) return;
-    if (variant == 1) {
-        const unsigned nb = (unsigned)(n / (256 * 4));
-        k_gather_slab<256, 4, 4160><<<nb, 256, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                       do_drift, pf);
-    } else {
-        const unsigned nb = (unsigned)(n / (512 * 4));
-        k_gather_slab<512, 4, 6656><<<nb, 512, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                       do_drift, pf);
-    }
+    if (variant == 1) gather_launch<256, 4160>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, s);
+    else gather_launch<512, 6656>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, s);
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {

@@ -206,16 +206,20 @@ def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
         F = m.get_force()
         dT, fs, pf = 0.0123, 1.87, 0.75
         vref = orc.kick_split(ng, cell, off, vel, F, dT, fs)
-        # kick alone, with every gather workgroup shape
+        # kick alone, with every gather workgroup shape, staging from the deposit's cell
+        # boxes and (after a zero drift that invalidates them) from its own reduction
         for gt in (1, 0):
-            m.set_tunable("gather_tile", gt)
-            m.set_particles_split(cell, off, vel)
-            m.cic()
-            m.solve_gradient()
-            m.update_velocities(dT, fs)
-            c1, o1, v1 = m.get_particles_split()
-            assert np.array_equal(v1, vref), gt
-            assert np.array_equal(c1, cell) and np.array_equal(o1, off)
+            for boxed in (True, False):
+                m.set_tunable("gather_tile", gt)
+                m.set_particles_split(cell, off, vel)
+                m.cic()
+                m.solve_gradient()
+                if not boxed:
+                    m.update_positions(0.0)
+                m.update_velocities(dT, fs)
+                c1, o1, v1 = m.get_particles_split()
+                assert np.array_equal(v1, vref), (gt, boxed)
+                assert np.array_equal(c1, cell) and np.array_equal(o1, off)
         # fused kick + drift from the same start
         m.set_particles_split(cell, off, vel)
         m.cic()
