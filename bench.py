@@ -140,6 +140,78 @@ def cpu_baseline(hg, threads, ng=128):
                       f"FP32 rounding, FP64 FFT), {el:.1f} s"}
 
 
+def ensemble_boxes(hg, per_class=32, sizes=(64, 128, 192), z_ini=30.0, n_steps=40):
+    """The batched-ensemble workload: per_class boxes at each size (grid = particles per
+    dimension, 2 Mpc/h per cell), seed 1000+b, sigma_8 drawn uniformly in [0.7, 0.9]
+    by a generator seeded 4242, z_ini -> 0 in n_steps."""
+    rng = np.random.default_rng(4242)
+    ps = []
+    for n in sizes:
+        for _ in range(per_class):
+            b = len(ps)
+            ps.append(hg.make_params(n, n, 2.0 * n, z_ini, 0.0, n_steps, seed=1000 + b,
+                                     ss8=float(rng.uniform(0.7, 0.9))))
+    return ps
+
+
+def ic_args(hg, p):
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "eh.ipk")
+        hg.write_ipk_eh(p, fn, 20.0, 20001)
+        hdr, vals = hg.read_ipk(fn)
+    a = 1.0 / (1.0 + p.z_ini)
+    delta, _ = hg.growth(p, a)
+    _, dot_delta = hg.growth(p, a - ts.deltaT / 2.0)
+    return hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT
+
+
+def run_ensemble(hg, steps, warmup, per_class):
+    """Batched ensemble vs the same boxes launched one at a time (one context per box,
+    each box's step completed before the next box's starts)."""
+    ps = ensemble_boxes(hg, per_class)
+    n_total = warmup + steps
+    ics = [ic_args(hg, p) for p in ps]
+    scheds = [hg.schedule(p, n_total, merge=True) for p in ps]
+    nparts = sum(p.np ** 3 for p in ps)
+    with hg.Ensemble(ps) as e:
+        e.autotune(2)
+        for b, a in enumerate(ics):
+            e.generate_ic(b, *a)
+        for s in range(warmup):
+            e.step([sc[s] for sc in scheds])
+        e.synchronize()
+        t0 = time.perf_counter()
+        for s in range(warmup, n_total):
+            e.step([sc[s] for sc in scheds])
+        e.synchronize()
+        t_batched = time.perf_counter() - t0
+        nbatches, _ = e.batches()
+        dev = e.device_bytes()
+    t_single = 0.0
+    for b, p in enumerate(ps):
+        with hg.MemoryManager(p) as m:
+            m.generate_ic(*ics[b])
+            for s in range(warmup):
+                m.step(scheds[b][s])
+            m.synchronize()
+            t0 = time.perf_counter()
+            for s in range(warmup, n_total):
+                m.step(scheds[b][s])
+                m.synchronize()
+            t_single += time.perf_counter() - t0
+    return {"metric": "aggregate particle-steps/s, batched ensemble vs one box at a time",
+            "value": nparts * steps / t_batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": t_batched * 1e3 / steps, "higher_is_better": True,
+            "one_at_a_time": {"value": nparts * steps / t_single, "ms_per_step": t_single * 1e3 / steps},
+            "speedup": t_single / t_batched, "batches": nbatches, "device_GB": dev / 1e9,
+            "dtype": "f32", "data": "synthetic: Zel'dovich ICs, Eisenstein-Hu no-wiggle P(k), per-box seed and sigma_8",
+            "config": {"workload": f"ensemble_{per_class}x(64^3,128^3,192^3)_box2Mpc_per_cell_z30",
+                       "boxes": len(ps), "particles": nparts}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +227,9 @@ def main():
                     help="after the run, copy MIB MiB device-to-device twice (a plain copy kernel of known "
                          "size, the reference point for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--gather-tile", type=int, default=-1, help="gather workgroup shape (library default: -1)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the batched-ensemble workload instead (its own JSON line)")
+    ap.add_argument("--per-class", type=int, default=32, help="ensemble boxes per size class")
     ap.add_argument("--only-copy", action="store_true",
                     help="run only the --copy-calibration copy (a separate profiler pass)")
     args = ap.parse_args()
@@ -173,6 +248,9 @@ def main():
         sys.exit(2)
     import hacc_gpm_amd as hg
 
+    if args.ensemble:
+        print(json.dumps(run_ensemble(hg, args.steps, args.warmup, args.per_class)))
+        return
     ng = args.ng
     workload = f"pm_step_{ng}^3_particles_{ng}^3_grid_box{int(args.rl)}"
     p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221)

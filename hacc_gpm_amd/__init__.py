@@ -133,6 +133,19 @@ def lib():
             "hgpm_set_tunable": (c_int, [vp, c_char_p, c_int]),
             "hgpm_get_tunable": (c_int, [vp, c_char_p, P(c_int)]),
             "hgpm_autotune": (c_int, [vp, c_int, vp, vp]),
+            "hgpm_ensemble_create": (c_int, [c_int, vp, P(vp)]),
+            "hgpm_ensemble_destroy": (c_int, [vp]),
+            "hgpm_ensemble_last_error": (c_char_p, [vp]),
+            "hgpm_ensemble_batches": (c_int, [vp, P(c_int), vp]),
+            "hgpm_ensemble_device_bytes": (c_int, [vp, P(ctypes.c_size_t)]),
+            "hgpm_ensemble_generate_ic": (c_int, [vp, c_int, vp, c_int, c_double, c_double, c_double, c_double,
+                                                  c_double, c_double]),
+            "hgpm_ensemble_set_particles": (c_int, [vp, c_int, vp, vp]),
+            "hgpm_ensemble_get_particles": (c_int, [vp, c_int, vp, vp]),
+            "hgpm_ensemble_power_spectrum": (c_int, [vp, c_int, c_int, vp, vp, vp]),
+            "hgpm_ensemble_step": (c_int, [vp, vp]),
+            "hgpm_ensemble_synchronize": (c_int, [vp]),
+            "hgpm_ensemble_autotune": (c_int, [vp, c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -443,6 +456,90 @@ class MemoryManager:
         self._c(lib().hgpm_get_timings(self.h, names, 8192, _ptr(ms), _ptr(cnt), 128, ctypes.byref(n)))
         parts = names.raw.split(b"\0")
         return {parts[i].decode(): (float(ms[i]), int(cnt[i])) for i in range(n.value)}
+
+
+# -------------------------------------------------------------- ensemble ----
+class Ensemble:
+    """Many boxes stepped together (hgpm_ensemble_*): boxes with the same dynamics are
+    batched into one set of launches per step; seeds and spectra stay per box."""
+
+    def __init__(self, params_list):
+        self.params = list(params_list)
+        arr = (Params * len(self.params))(*self.params)
+        h = ctypes.c_void_p()
+        rc = lib().hgpm_ensemble_create(len(self.params), arr, ctypes.byref(h))
+        if rc:
+            raise HgpmError(rc, lib().hgpm_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hgpm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        if rc:
+            raise HgpmError(rc, lib().hgpm_ensemble_last_error(self.h).decode())
+
+    def batches(self):
+        n = ctypes.c_int()
+        of = np.zeros(len(self.params), np.int32)
+        self._c(lib().hgpm_ensemble_batches(self.h, ctypes.byref(n), _ptr(of)))
+        return n.value, of
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        self._c(lib().hgpm_ensemble_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def generate_ic(self, box, ipk_header, ipk_values, delta, dot_delta, fscal, deltaT):
+        v = np.ascontiguousarray(ipk_values, np.float64)
+        self._c(lib().hgpm_ensemble_generate_ic(self.h, int(box), _ptr(v), v.size, float(ipk_header[3]),
+                                                float(ipk_header[1]), float(delta), float(dot_delta), float(fscal),
+                                                float(deltaT)))
+
+    def set_particles(self, box, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1)
+        n = self.params[box].np ** 3
+        assert pos.size == 3 * n and vel.size == 3 * n
+        self._c(lib().hgpm_ensemble_set_particles(self.h, int(box), _ptr(pos), _ptr(vel)))
+
+    def get_particles(self, box):
+        n = self.params[box].np ** 3
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        self._c(lib().hgpm_ensemble_get_particles(self.h, int(box), _ptr(pos), _ptr(vel)))
+        return pos, vel
+
+    def power_spectrum(self, box, fold=0):
+        nb = self.params[box].pk_bins
+        k, pk, cnt = np.zeros(nb), np.zeros(nb), np.zeros(nb, np.int64)
+        self._c(lib().hgpm_ensemble_power_spectrum(self.h, int(box), int(fold), _ptr(k), _ptr(pk), _ptr(cnt)))
+        return k, pk, cnt
+
+    def step(self, coeffs):
+        """coeffs: one StepCoeffs per box (equal within a batch)."""
+        arr = (StepCoeffs * len(self.params))(*coeffs)
+        self._c(lib().hgpm_ensemble_step(self.h, arr))
+
+    def synchronize(self):
+        self._c(lib().hgpm_ensemble_synchronize(self.h))
+
+    def autotune(self, reps=3):
+        self._c(lib().hgpm_ensemble_autotune(self.h, int(reps)))
 
 
 # ------------------------------------------------- reference-style API ----

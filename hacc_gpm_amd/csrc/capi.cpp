@@ -32,7 +32,8 @@ struct PendingEv {
 struct hgpm_ctx {
     hgpm_params p{};
     int ng = 0, np = 0, prec = HGPM_FP32;
-    int64_t N = 0, G = 0, Gic = 0;
+    int64_t N = 0, G = 0, Gic = 0; // per box
+    int nbox = 1; // FP32 batches: boxes of the same shape stored back to back
     hipStream_t st = nullptr;
     int device = 0;
     std::string err;
@@ -46,8 +47,9 @@ struct hgpm_ctx {
     float2* B = nullptr;             // int64 density / Fx^+iFy^ / (Fx,Fy) (8 B per cell)
     float* fz = nullptr;             // Fz (4 B per cell)
     float2 *h2 = nullptr, *h3 = nullptr; // half spectra Fx^, Fy^ of the solve (A holds rho^ / Fz^)
-    unsigned long long* ctl = nullptr;   // deposit control words; ctl[0] = deposited total
-    unsigned long long* total = nullptr; // = ctl
+    unsigned long long* ctl = nullptr;    // deposit control words (shared by the boxes)
+    unsigned long long* totals = nullptr; // deposited mass per box
+    unsigned long long* total = nullptr;  // = totals (of the viewed box, see BoxView)
     // deposit arena route (k_particles.hip): window arena, per-owner-block chunk slots
     unsigned long long* arena = nullptr;
     unsigned long long arena_alloc = 0, arena_cap = 0; // cells allocated / usable
@@ -228,8 +230,10 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.ctl = c->ctl;
     b.tile_total = c->tile_total;
     b.boxes = c->boxes;
+    b.nbox = c->nbox;
+    b.totals = c->total;
     // about 256 tiles per sub-arena, at most HGPM_DEP_NSUB sub-arenas
-    const int64_t nt = c->N / (16 * 16 * 16);
+    const int64_t nt = c->N * c->nbox / (16 * 16 * 16);
     b.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(HGPM_DEP_NSUB, nt / 256));
     return b;
 }
@@ -242,7 +246,7 @@ int f32_deposit(hgpm_ctx* c) {
     launch_zero_u64(c->ctl, HGPM_DEP_CTL, c->st);
     {
         Timed t1(c, "deposit_windows");
-        launch_deposit_windows(c->pos, c->N, c->ng, ref_mass(c), b, c->st);
+        launch_deposit_windows(c->pos, c->N * c->nbox, c->ng, ref_mass(c), b, c->st);
         c->boxes_valid = true;
     }
     {
@@ -251,7 +255,7 @@ int f32_deposit(hgpm_ctx* c) {
     }
     {
         Timed t3(c, "deposit_deferred");
-        launch_deposit_deferred(c->pos, c->N, c->ng, ref_mass(c), (long long*)c->B, b, c->st);
+        launch_deposit_deferred(c->pos, c->N * c->nbox, c->ng, ref_mass(c), (long long*)c->B, b, c->st);
     }
     LAUNCHCHK(c);
     return HGPM_OK;
@@ -264,29 +268,29 @@ int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {
         Timed t(c, "fft_fwd_z_r2c");
-        f->fwd_z_r2c((const long long*)c->B, c->total, c->A, c->twf, c->st);
+        f->fwd_z_r2c((const long long*)c->B, c->total, c->A, c->twf, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_fwd_y");
-        f->pass_half(c->A, 1, -1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->A, 1, -1, c->twf, c->var_cols, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_x_solve");
-        f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->st);
+        f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_inv_y");
-        f->pass_half(c->h2, 1, +1, c->twf, c->var_cols, c->st);
-        f->pass_half(c->h3, 1, +1, c->twf, c->var_cols, c->st);
-        f->pass_half(c->A, 1, +1, c->twf, c->var_cols, c->st);
+        f->pass_half(c->h2, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
+        f->pass_half(c->h3, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
+        f->pass_half(c->A, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fxy");
-        f->inv_z_c2r_xy(c->h2, c->h3, c->B, c->twf, c->st);
+        f->inv_z_c2r_xy(c->h2, c->h3, c->B, c->twf, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_inv_z_fz");
-        f->inv_z_c2r_pair(c->A, c->sk, c->fz, c->twf, c->st);
+        f->inv_z_c2r_pair(c->A, c->sk, c->fz, c->twf, c->nbox, c->st);
     }
     LAUNCHCHK(c);
     return HGPM_OK;
@@ -378,8 +382,11 @@ const char* hgpm_version(void) { return "hgpm 0.1.0 (gfx950)"; }
 
 const char* hgpm_last_error(const hgpm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
 
-int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
+namespace {
+int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
     if (!p || !out) return fail(nullptr, HGPM_ERR_ARG, "hgpm_create: null argument");
+    if (nbox < 1 || (nbox > 1 && p->precision != HGPM_FP32))
+        return fail(nullptr, HGPM_ERR_ARG, "batches of boxes need FP32 and at least one box");
     *out = nullptr;
     if (!supported_n(p->ng) || !supported_n(p->np))
         return fail(nullptr, HGPM_ERR_ARG, "unsupported NG/NP " + std::to_string(p->ng) + "/" + std::to_string(p->np) +
@@ -403,6 +410,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
     c->N = (int64_t)p->np * p->np * p->np;
     c->G = (int64_t)p->ng * p->ng * p->ng;
     c->Gic = c->G; // np <= ng
+    c->nbox = nbox;
     c->fft = fft_ops(p->ng);
     c->fftic = fft_ops(p->np);
     (void)hipGetDevice(&c->device);
@@ -419,23 +427,26 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
     }
     const size_t N = (size_t)c->N, G = (size_t)c->G;
     if (c->prec == HGPM_FP32) {
-        if ((rc = dalloc(c, &c->pos, sizeof(PosSplitH) * N, "pos")) ||
-            (rc = dalloc(c, &c->vel, sizeof(float) * 3 * N, "vel")) ||
-            (rc = dalloc(c, &c->A, sizeof(float2) * G, "grid A")) ||
-            (rc = dalloc(c, &c->B, sizeof(float2) * G, "grid B")) ||
-            (rc = dalloc(c, &c->fz, sizeof(float) * G, "Fz")) ||
-            (rc = dalloc(c, &c->h2, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz, "half spectrum Fx")) ||
-            (rc = dalloc(c, &c->h3, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz, "half spectrum Fy")) ||
-            (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")))
+        // every per-box buffer holds nbox boxes back to back
+        const size_t NB = N * nbox, GB = G * nbox;
+        if ((rc = dalloc(c, &c->pos, sizeof(PosSplitH) * NB, "pos")) ||
+            (rc = dalloc(c, &c->vel, sizeof(float) * 3 * NB, "vel")) ||
+            (rc = dalloc(c, &c->A, sizeof(float2) * GB, "grid A")) ||
+            (rc = dalloc(c, &c->B, sizeof(float2) * GB, "grid B")) ||
+            (rc = dalloc(c, &c->fz, sizeof(float) * GB, "Fz")) ||
+            (rc = dalloc(c, &c->h2, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz * nbox, "half spectrum Fx")) ||
+            (rc = dalloc(c, &c->h3, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz * nbox, "half spectrum Fy")) ||
+            (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")) ||
+            (rc = dalloc(c, &c->totals, sizeof(unsigned long long) * nbox, "deposit totals")))
             return bail(rc);
-        c->total = c->ctl;
+        c->total = c->totals;
         {
             // window arena: 4 cells per grid cell holds every tile window of the runs
             // measured here through z = 0 (the windows' total grows from ~1.4 G to ~4 G
             // cells as structure forms); tiles beyond it take the atomic route
-            const size_t nblk = (size_t)(p->ng / 16) * (p->ng / 16) * (p->ng / 16);
-            const size_t ntiles = N / (16 * 16 * 16);
-            c->arena_alloc = 4ull * G;
+            const size_t nblk = (size_t)(p->ng / 16) * (p->ng / 16) * (p->ng / 16) * nbox;
+            const size_t ntiles = N * nbox / (16 * 16 * 16);
+            c->arena_alloc = 4ull * G * nbox;
             c->arena_cap = c->arena_alloc;
             if ((rc = dalloc(c, &c->arena, sizeof(unsigned long long) * c->arena_alloc, "deposit arena")) ||
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
@@ -505,11 +516,14 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) {
     *out = c;
     return HGPM_OK;
 }
+} // namespace
+
+int hgpm_create(const hgpm_params* p, hgpm_ctx** out) { return ctx_create(p, 1, out); }
 
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->sk, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->totals, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->sk, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -747,7 +761,7 @@ int hgpm_update_positions(hgpm_ctx* c, double prefactor) {
     if (c->prec == HGPM_FP32) {
         c->boxes_valid = false;
         Timed t(c, "drift");
-        launch_drift(c->pos, c->vel, c->N, c->ng, (float)prefactor, c->st);
+        launch_drift(c->pos, c->vel, c->N * c->nbox, c->ng, (float)prefactor, c->st);
     } else {
         Timed t(c, "f64_drift");
         launch_f64_drift(c->pos64, c->vel64, c->N, c->ng, (double)(float)prefactor, c->st);
@@ -875,8 +889,8 @@ int hgpm_update_velocities(hgpm_ctx* c, double deltaT, double fscal) {
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick");
-        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather,
-                           c->boxes_valid ? c->boxes : nullptr, c->st);
+        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather,
+                           c->boxes_valid ? c->boxes : nullptr, c->nbox, c->st);
     } else {
         Timed t(c, "f64_icic");
         launch_f64_icic(c->pos64, c->vel64, c->grad64, c->N, c->ng, deltaT, fscal, c->st);
@@ -892,8 +906,8 @@ int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) 
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick_drift");
-        launch_gather_kick(c->pos, c->vel, c->N, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor, c->var_gather,
-                           c->boxes_valid ? c->boxes : nullptr, c->st);
+        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor,
+                           c->var_gather, c->boxes_valid ? c->boxes : nullptr, c->nbox, c->st);
         c->boxes_valid = false;
         LAUNCHCHK(c);
         return HGPM_OK;
@@ -1133,8 +1147,8 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
-    launch_zero_u64((unsigned long long*)c->A, (size_t)c->G, c->st);
-    launch_zero_u64((unsigned long long*)c->B, (size_t)c->G, c->st);
+    launch_zero_u64((unsigned long long*)c->A, (size_t)c->G * c->nbox, c->st);
+    launch_zero_u64((unsigned long long*)c->B, (size_t)c->G * c->nbox, c->st);
     double best_c = 1e30, best_x = 1e30;
     for (int v = 0; v < f->n_variants; v++) {
         if (!((f->valid_mask >> v) & 1u)) {
@@ -1144,8 +1158,8 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
         }
         for (int kind = 0; kind < 2; kind++) {
             auto run = [&]() {
-                if (kind == 0) f->pass_half(c->A, 1, -1, c->twf, v, c->st);
-                else f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, v, c->st);
+                if (kind == 0) f->pass_half(c->A, 1, -1, c->twf, v, c->nbox, c->st);
+                else f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, v, c->nbox, c->st);
             };
             run();
             HIPCHK(c, hipEventRecord(a, c->st));
@@ -1168,6 +1182,220 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     c->have_density = c->have_force = false;
+    return HGPM_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------- ensembles ----
+// Many boxes stepped together.  Boxes whose dynamics agree (grid, particles, box size,
+// time range and steps, cosmology other than sigma_8 and n_s, P(k) binning) form one
+// batch: a context holding them back to back, stepped by the same launches (every
+// kernel runs over all particles / rows / owner blocks of the batch).  Each batch has
+// its own stream, so batches of different sizes also overlap.  Seeds and spectra stay
+// per box (initial conditions are generated box by box).
+struct hgpm_ensemble {
+    std::vector<hgpm_ctx*> batch;
+    std::vector<int> batch_of, slot_of; // per box
+    std::vector<int> first;             // per batch: its first box
+    std::vector<hgpm_params> params;    // per box
+    std::string err;
+};
+
+namespace {
+// Box b of a batch seen as a single-box context: the per-box buffers moved to its
+// slice, its own parameters (seed, spectrum), nbox = 1; restored on destruction.  A
+// deposit made through the view rewrites the shared cell boxes of the gather, which
+// are then invalid for the batch.
+struct BoxView {
+    hgpm_ctx* c;
+    hgpm_params p;
+    PosSplitH* pos;
+    float *vel, *fz;
+    float2 *A, *B, *h2, *h3;
+    unsigned long long* total;
+    int nbox;
+    BoxView(hgpm_ctx* c_, int b, const hgpm_params& bp) : c(c_) {
+        p = c->p; pos = c->pos; vel = c->vel; fz = c->fz; A = c->A; B = c->B; h2 = c->h2; h3 = c->h3;
+        total = c->total; nbox = c->nbox;
+        const int64_t hn = (int64_t)c->ng * c->ng * c->fft->hz;
+        c->p = bp;
+        c->pos += (int64_t)b * c->N;
+        c->vel += 3 * (int64_t)b * c->N;
+        c->fz += (int64_t)b * c->G;
+        c->A += (int64_t)b * c->G;
+        c->B += (int64_t)b * c->G;
+        c->h2 += (int64_t)b * hn;
+        c->h3 += (int64_t)b * hn;
+        c->total += b;
+        c->nbox = 1;
+    }
+    ~BoxView() {
+        c->p = p; c->pos = pos; c->vel = vel; c->fz = fz; c->A = A; c->B = B; c->h2 = h2; c->h3 = h3;
+        c->total = total; c->nbox = nbox;
+        c->boxes_valid = false;
+    }
+};
+
+bool same_dynamics(const hgpm_params& a, const hgpm_params& b) {
+    return a.ng == b.ng && a.np == b.np && a.rl == b.rl && a.z_ini == b.z_ini && a.z_fin == b.z_fin &&
+           a.nsteps == b.nsteps && a.hubble == b.hubble && a.omega_cdm == b.omega_cdm && a.omega_nu == b.omega_nu &&
+           a.deut == b.deut && a.w_de == b.w_de && a.wa_de == b.wa_de && a.t_cmb == b.t_cmb &&
+           a.neff_massless == b.neff_massless && a.neff_massive == b.neff_massive && a.pk_bins == b.pk_bins &&
+           a.pk_folds == b.pk_folds && a.precision == b.precision;
+}
+
+int efail(hgpm_ensemble* e, int code, const std::string& m) {
+    if (e) e->err = m;
+    else set_global_error(m);
+    return code;
+}
+
+int ebox(hgpm_ensemble* e, int box) {
+    if (!e) return HGPM_ERR_ARG;
+    if (box < 0 || box >= (int)e->params.size()) return efail(e, HGPM_ERR_ARG, "box index out of range");
+    return HGPM_OK;
+}
+
+// Forwards a context failure of box b's batch to the ensemble's message.
+int eforward(hgpm_ensemble* e, hgpm_ctx* c, int rc) {
+    if (rc != HGPM_OK) e->err = c->err;
+    return rc;
+}
+} // namespace
+
+extern "C" {
+
+int hgpm_ensemble_create(int n_boxes, const hgpm_params* params, hgpm_ensemble** out) {
+    if (!params || !out || n_boxes < 1) return efail(nullptr, HGPM_ERR_ARG, "hgpm_ensemble_create: bad arguments");
+    *out = nullptr;
+    hgpm_ensemble* e = new hgpm_ensemble();
+    e->params.assign(params, params + n_boxes);
+    e->batch_of.assign(n_boxes, -1);
+    e->slot_of.assign(n_boxes, -1);
+    std::vector<std::vector<int>> members;
+    for (int b = 0; b < n_boxes; b++) {
+        if (params[b].precision != HGPM_FP32) {
+            delete e;
+            return efail(nullptr, HGPM_ERR_UNSUPPORTED, "ensembles run the FP32 engine");
+        }
+        int g = -1;
+        for (size_t k = 0; k < members.size(); k++)
+            if (same_dynamics(params[members[k][0]], params[b])) { g = (int)k; break; }
+        if (g < 0) {
+            g = (int)members.size();
+            members.emplace_back();
+        }
+        e->batch_of[b] = g;
+        e->slot_of[b] = (int)members[g].size();
+        members[g].push_back(b);
+    }
+    for (auto& m : members) {
+        hgpm_ctx* c = nullptr;
+        int rc = ctx_create(&params[m[0]], (int)m.size(), &c);
+        if (rc != HGPM_OK) {
+            for (hgpm_ctx* q : e->batch) hgpm_destroy(q);
+            delete e;
+            return rc; // ctx_create set the global message
+        }
+        e->batch.push_back(c);
+        e->first.push_back(m[0]);
+    }
+    *out = e;
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_destroy(hgpm_ensemble* e) {
+    if (!e) return HGPM_OK;
+    for (hgpm_ctx* c : e->batch) hgpm_destroy(c);
+    delete e;
+    return HGPM_OK;
+}
+
+const char* hgpm_ensemble_last_error(const hgpm_ensemble* e) { return e ? e->err.c_str() : global_error(); }
+
+int hgpm_ensemble_batches(const hgpm_ensemble* e, int* n_batches, int* batch_of_box) {
+    if (!e || !n_batches) return HGPM_ERR_ARG;
+    *n_batches = (int)e->batch.size();
+    if (batch_of_box)
+        for (size_t b = 0; b < e->batch_of.size(); b++) batch_of_box[b] = e->batch_of[b];
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_device_bytes(const hgpm_ensemble* e, size_t* bytes) {
+    if (!e || !bytes) return HGPM_ERR_ARG;
+    size_t t = 0;
+    for (hgpm_ctx* c : e->batch) t += c->bytes;
+    *bytes = t;
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_generate_ic(hgpm_ensemble* e, int box, const double* ipk, int ipk_bins, double ipk_k_delta,
+                              double ipk_k_max, double delta, double dot_delta, double fscal, double deltaT) {
+    int rc = ebox(e, box);
+    if (rc) return rc;
+    hgpm_ctx* c = e->batch[e->batch_of[box]];
+    BoxView v(c, e->slot_of[box], e->params[box]);
+    return eforward(e, c, hgpm_generate_ic(c, ipk, ipk_bins, ipk_k_delta, ipk_k_max, delta, dot_delta, fscal, deltaT));
+}
+
+int hgpm_ensemble_set_particles(hgpm_ensemble* e, int box, const float* pos, const float* vel) {
+    int rc = ebox(e, box);
+    if (rc) return rc;
+    hgpm_ctx* c = e->batch[e->batch_of[box]];
+    BoxView v(c, e->slot_of[box], e->params[box]);
+    return eforward(e, c, hgpm_set_particles(c, pos, vel));
+}
+
+int hgpm_ensemble_get_particles(hgpm_ensemble* e, int box, float* pos, float* vel) {
+    int rc = ebox(e, box);
+    if (rc) return rc;
+    hgpm_ctx* c = e->batch[e->batch_of[box]];
+    BoxView v(c, e->slot_of[box], e->params[box]);
+    return eforward(e, c, hgpm_get_particles(c, pos, vel));
+}
+
+int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k, double* pk, int64_t* count) {
+    int rc = ebox(e, box);
+    if (rc) return rc;
+    hgpm_ctx* c = e->batch[e->batch_of[box]];
+    BoxView v(c, e->slot_of[box], e->params[box]);
+    return eforward(e, c, hgpm_power_spectrum(c, fold, k, pk, count));
+}
+
+// One step of every box; coeffs[b] for box b.  The boxes of a batch share one set of
+// coefficients (their timesteppers agree), so coeffs must agree within a batch.
+int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs) {
+    if (!e || !coeffs) return efail(e, HGPM_ERR_ARG, "hgpm_ensemble_step: null argument");
+    for (size_t b = 0; b < e->params.size(); b++) {
+        const hgpm_step_coeffs& a = coeffs[e->first[e->batch_of[b]]];
+        const hgpm_step_coeffs& q = coeffs[b];
+        if (a.drift0 != q.drift0 || a.kick_dt != q.kick_dt || a.kick_fscal != q.kick_fscal || a.drift1 != q.drift1 ||
+            a.skip_drift0 != q.skip_drift0)
+            return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch (box " + std::to_string(b) + ")");
+    }
+    for (size_t g = 0; g < e->batch.size(); g++) {
+        int rc = hgpm_step(e->batch[g], &coeffs[e->first[g]]);
+        if (rc) return eforward(e, e->batch[g], rc);
+    }
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_synchronize(hgpm_ensemble* e) {
+    if (!e) return HGPM_ERR_ARG;
+    for (hgpm_ctx* c : e->batch) {
+        int rc = hgpm_synchronize(c);
+        if (rc) return eforward(e, c, rc);
+    }
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_autotune(hgpm_ensemble* e, int reps) {
+    if (!e) return HGPM_ERR_ARG;
+    for (hgpm_ctx* c : e->batch) {
+        int rc = hgpm_autotune(c, reps, nullptr, nullptr);
+        if (rc) return eforward(e, c, rc);
+    }
     return HGPM_OK;
 }
 

@@ -18,16 +18,19 @@ struct FftOps {
     void (*pass_f64)(double2* d, int axis, int dir, const double2* tw, hipStream_t s);
     void (*fwd_z_density)(const long long* rho, const unsigned long long* total, float2* out,
                           const float2* tw, hipStream_t s);
-    // half-spectrum solve (rows of hz = n/2 + 16 complex, k_z = 0..n/2)
+    // half-spectrum solve (rows of hz = n/2 + 16 complex, k_z = 0..n/2) on nbox grids of
+    // the same shape stored back to back; totals[b] is box b's deposited mass
     int hz;
-    void (*pass_half)(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s);
-    void (*fwd_z_r2c)(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
-                      hipStream_t s);
+    void (*pass_half)(float2* d, int axis, int dir, const float2* tw, int variant, int nbox, hipStream_t s);
+    void (*fwd_z_r2c)(const long long* rho, const unsigned long long* totals, float2* out, const float2* tw,
+                      int nbox, hipStream_t s);
     void (*x_solve_r2c)(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr,
-                        const float* sk, const float* hs, int variant, hipStream_t s);
-    void (*inv_z_c2r_xy)(const float2* hx, const float2* hy, float2* fxy, const float2* tw, hipStream_t s);
+                        const float* sk, const float* hs, int variant, int nbox, hipStream_t s);
+    void (*inv_z_c2r_xy)(const float2* hx, const float2* hy, float2* fxy, const float2* tw, int nbox,
+                         hipStream_t s);
     // Fz from U = i g rho^ (half spectra after the inverse x, y passes): Fz^ = sk[k_z] U
-    void (*inv_z_c2r_pair)(const float2* hz, const float* sk, float* fz, const float2* tw, hipStream_t s);
+    void (*inv_z_c2r_pair)(const float2* hz, const float* sk, float* fz, const float2* tw, int nbox,
+                           hipStream_t s);
 };
 
 #define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
@@ -72,6 +75,8 @@ struct DepositBuffers {
     unsigned long long* tile_total;
     int nsub; // sub-arenas in use, 1..HGPM_DEP_NSUB
     void* boxes; // 32 B per half tile: the cell box the gather stages (written by the windows pass)
+    int nbox;    // boxes of the same shape stored back to back (particles, grids, owner blocks)
+    unsigned long long* totals; // deposited mass per box
 };
 void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
@@ -83,7 +88,7 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float ma
 // since, else nullptr (the gather reduces them itself)
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
-                        hipStream_t s);
+                        int nbox, hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
                               int ng, hipStream_t s);

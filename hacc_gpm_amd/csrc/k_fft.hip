@@ -74,9 +74,12 @@ __global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __re
     // adjacent z-chunks of one row share 128-byte lines: keep them on one XCD's L2
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int nzc = pz / CC;
+    // outer runs over (box, x) for the y pass and (box, y) for the x pass: a batch holds
+    // several grids of the same shape back to back
     const int outer = tile / nzc, zc = tile % nzc;
     const size_t NP = (size_t)N * pz;
-    const size_t base = axis == 1 ? (size_t)outer * NP + (size_t)zc * CC : (size_t)outer * pz + (size_t)zc * CC;
+    const size_t base = axis == 1 ? (size_t)outer * NP + (size_t)zc * CC
+                                  : (size_t)(outer / N) * N * NP + (size_t)(outer % N) * pz + (size_t)zc * CC;
     const size_t stride = axis == 1 ? (size_t)pz : NP;
     fft_tile<V, N, CC, NTC, DIR, false, false>(
         [&](int, int, int n, int c) { return d[base + n * stride + c]; },
@@ -96,7 +99,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __rest
     __shared__ float2 lds[lds_elems<N, PC>()];
     float2* spec = lds; // the last stage has read all its inputs before the first output
     const size_t row0 = (size_t)blockIdx.x * RC;
-    const double mean = (double)(long long)(*total) / ((double)N * N * N);
+    // total of this row's box: a batch holds several boxes of the same shape
+    const double mean = (double)(long long)total[row0 / ((size_t)N * N)] / ((double)N * N * N);
     fft_tile<float2, N, PC, NT, -1, false, true>(
         [&](int, int, int n, int c) {
             const double qa = (double)rho[(row0 + 2 * c) * N + n];
@@ -135,9 +139,10 @@ __global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, floa
                                                      const float* __restrict hs, float gcoef) {
     __shared__ float2 lds[lds_elems<N, CC>()];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
-    const int y = tile / (HZ / CC), zc = tile % (HZ / CC);
+    const int outer = tile / (HZ / CC), zc = tile % (HZ / CC); // outer = (box, y)
+    const int y = outer % N;
     const size_t NP = (size_t)N * HZ;
-    const size_t base = (size_t)y * HZ + (size_t)zc * CC;
+    const size_t base = (size_t)(outer / N) * N * NP + (size_t)y * HZ + (size_t)zc * CC;
     typedef LastShape<N, CC, NTC> LS;
     typedef StageShape<N, CC, NTC, LS::R> SS;
     float2 R[LS::NB][LS::R];
@@ -230,8 +235,8 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_pair(const float2* __restrict h
 // means fewer elements, and registers, per thread: the fused x-solve holds the
 // spectrum between its three transforms and needs them.
 template <typename V, int CC, int NTC>
-void cols_launch(V* d, int axis, int dir, const V* tw, int pz, hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * (pz / CC));
+void cols_launch(V* d, int axis, int dir, const V* tw, int pz, hipStream_t s, int nbox = 1) {
+    unsigned nb = (unsigned)((size_t)nbox * N * (pz / CC));
     if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
     else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
 }
@@ -272,11 +277,11 @@ void pass_f32(float2* d, int axis, int dir, const float2* tw, int variant, hipSt
         cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, N, s);
     });
 }
-// Column pass over a half-spectrum grid N x N x HZ.
-void pass_half(float2* d, int axis, int dir, const float2* tw, int variant, hipStream_t s) {
+// Column pass over nbox half-spectrum grids N x N x HZ.
+void pass_half(float2* d, int axis, int dir, const float2* tw, int variant, int nbox, hipStream_t s) {
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, HZ, s);
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, HZ, s, nbox);
     });
 }
 void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
@@ -288,32 +293,34 @@ void fwd_z_density(const long long* rho, const unsigned long long* total, float2
     unsigned nb = (unsigned)((size_t)N * N / rc32());
     k_rows_density<rc32()><<<nb, NT, 0, s>>>(rho, total, out, tw);
 }
-void fwd_z_r2c(const long long* rho, const unsigned long long* total, float2* out, const float2* tw, hipStream_t s) {
+// nbox grids of the same shape back to back (rows of every box in one launch)
+void fwd_z_r2c(const long long* rho, const unsigned long long* totals, float2* out, const float2* tw, int nbox,
+               hipStream_t s) {
     constexpr int RC = 2 * rc32();
-    unsigned nb = (unsigned)((size_t)N * N / RC);
-    k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, total, out, tw);
+    unsigned nb = (unsigned)((size_t)nbox * N * N / RC);
+    k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, totals, out, tw);
 }
-void inv_z_c2r_xy(const float2* hx, const float2* hy, float2* fxy, const float2* tw, hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * N / rc32());
+void inv_z_c2r_xy(const float2* hx, const float2* hy, float2* fxy, const float2* tw, int nbox, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)nbox * N * N / rc32());
     k_rows_c2r_xy<rc32()><<<nb, NT, 0, s>>>(hx, hy, fxy, tw);
 }
-void inv_z_c2r_pair(const float2* hz, const float* sk, float* fz, const float2* tw, hipStream_t s) {
+void inv_z_c2r_pair(const float2* hz, const float* sk, float* fz, const float2* tw, int nbox, hipStream_t s) {
     constexpr int RC = 2 * rc32();
-    unsigned nb = (unsigned)((size_t)N * N / RC);
+    unsigned nb = (unsigned)((size_t)nbox * N * N / RC);
     k_rows_c2r_pair<RC><<<nb, NT, 0, s>>>(hz, sk, fz, tw);
 }
 template <int CC, int NTC>
 void x_solve_r2c_launch(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
-                        const float* hs, hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)N * (HZ / CC));
+                        const float* hs, int nbox, hipStream_t s) {
+    unsigned nb = (unsigned)((size_t)nbox * N * (HZ / CC));
     const float gcoef = (float)(0.25 / ((double)N * N * N));
     k_x_solve_r2c<CC, NTC><<<nb, NTC, 0, s>>>(h1, h2, h3, twf, twr, sk, hs, gcoef);
 }
 void x_solve_r2c(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
-                 const float* hs, int variant, hipStream_t s) {
+                 const float* hs, int variant, int nbox, hipStream_t s) {
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        x_solve_r2c_launch<cc_of(v), nt_of(v)>(h1, h2, h3, twf, twr, sk, hs, s);
+        x_solve_r2c_launch<cc_of(v), nt_of(v)>(h1, h2, h3, twf, twr, sk, hs, nbox, s);
     });
 }
 void twiddles(double* fwd, double* rev) {

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
                                                             unsigned* __restrict deferred,
                                                             unsigned long long* __restrict ctl,
                                                             unsigned long long* __restrict tile_total, int nsub,
-                                                            int4* __restrict boxes) {
+                                                            int4* __restrict boxes, int ntb, int nblk) {
     __shared__ unsigned long long win[WMAX];
     __shared__ int red[12][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
@@ -358,10 +358,12 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
         s_state = st;
     }
     __syncthreads();
+    // owner blocks of this tile's box (a batch holds several boxes of the same shape)
+    const int blk0 = (tile / ntb) * nblk;
     if (s_state == 0) {
         for (int c = tid; c < nch; c += DEP_NT) {
             const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
-            const unsigned s = atomicAdd(&count[ch.blk], 1u);
+            const unsigned s = atomicAdd(&count[blk0 + ch.blk], 1u);
             s_slot[c] = (unsigned short)(s < (unsigned)nslots ? s : 0xFFFFu);
             if (s >= (unsigned)nslots) s_state = 2;
         }
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
                               (ch.m[2] - 1) << 20) |
                    (state == 2 ? 0x80000000u : 0u);
             de.w = (unsigned)e[1] | (unsigned)e[2] << 16;
-            slots[(size_t)ch.blk * nslots + s_slot[c]] = de;
+            slots[(size_t)(blk0 + ch.blk) * nslots + s_slot[c]] = de;
         }
     }
     if (state != 0) {
@@ -400,12 +402,15 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long long* __restrict arena,
                                                           const uint4* __restrict slots,
                                                           unsigned* __restrict count, int nslots, int ng,
-                                                          unsigned long long* __restrict rho) {
+                                                          unsigned long long* __restrict rho, int nblk,
+                                                          int64_t gcells) {
     __shared__ unsigned long long acc[OB * OB * OB];
     __shared__ uint4 desc[HGPM_DEP_SLOTS];
     __shared__ int s_n;
     const int nb = ng / OB;
-    const int blk = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int blk = (int)xcd_remap(blockIdx.x, gridDim.x); // over every box of the batch
+    const int box = blk / nblk, lb = blk - box * nblk;
+    rho += (int64_t)box * gcells;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid == 0) {
         const unsigned c = count[blk];
@@ -451,7 +456,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long lo
         }
     }
     __syncthreads();
-    const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+    const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
     for (int i = tid; i < OB * OB * OB; i += OWN_NT) {
         const int lx = i >> 8, ly = (i >> 4) & 15, lz = i & 15;
         rho[gidx(ng, bx * OB + lx, by * OB + ly, bz * OB + lz)] = acc[i];
@@ -464,7 +469,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
                                                              float mass, unsigned long long* __restrict rho,
                                                              const unsigned long long* __restrict ctl,
                                                              const unsigned* __restrict deferred,
-                                                             unsigned long long* __restrict tile_total) {
+                                                             unsigned long long* __restrict tile_total, int ntb,
+                                                             int64_t gcells) {
     __shared__ unsigned long long win[WMAX];
     __shared__ int red[12][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
@@ -476,25 +482,28 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
         DepRegs R;
         int lo[3], e[3];
         dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
-        const long long local = dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, rho, 0ull);
+        unsigned long long* box_rho = rho + (int64_t)(tile / ntb) * gcells;
+        const long long local = dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, box_rho, 0ull);
         dep_tile_total(local, s_tot, &tile_total[tile]);
         __syncthreads();
     }
 }
 
-// ctl[0] = sum of the tiles' totals (one workgroup; integer sum, order-free).
+// totals[b] = sum of the totals of box b's tiles (one workgroup per box; integer sum,
+// order-free).
 __global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long* __restrict tile_total,
-                                                        int64_t ntiles, unsigned long long* __restrict ctl) {
+                                                        int64_t ntb, unsigned long long* __restrict totals) {
     __shared__ unsigned long long s[1024 / 64];
     unsigned long long t = 0;
-    for (int64_t i = threadIdx.x; i < ntiles; i += 1024) t += tile_total[i];
+    tile_total += (int64_t)blockIdx.x * ntb;
+    for (int64_t i = threadIdx.x; i < ntb; i += 1024) t += tile_total[i];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long a = 0;
         for (int w = 0; w < 1024 / 64; w++) a += s[w];
-        ctl[0] = a;
+        totals[blockIdx.x] = a;
     }
 }
 
@@ -517,7 +526,8 @@ template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
 __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float2* __restrict fxy,
                                                        const float* __restrict fz, double deltaT, double fscal,
-                                                       int do_drift, float pf, const int4* __restrict boxes) {
+                                                       int do_drift, float pf, const int4* __restrict boxes,
+                                                       int64_t npb, int64_t gcells) {
     constexpr int GT_TILE = GT_NT * GT_PPT;
     static_assert(DEP_TILE % GT_TILE == 0 && GT_TILE <= DEP_TILE / 2, "tile");
     static_assert(GMAX >= 4 * 1025, "two rows of the widest window");
@@ -526,6 +536,12 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
     __shared__ int red[6][GT_NT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
+    {
+        // the force grids of this tile's box (a batch holds several boxes of the same shape)
+        const int64_t box = p0 / npb;
+        fxy += box * gcells;
+        fz += box * gcells;
+    }
     // BOXED: the deposit of these positions stored the box of each half deposit tile,
     // relative to the deposit tile's first particle; no reduction, and the staging
     // loads can start while the particle loads are in flight
@@ -803,53 +819,63 @@ void launch_drift(PosSplitH* pos, const float* vel, int64_t n, int ng, float pf,
     k_drift<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, vel, n, ng, pf);
 }
 void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
-                           hipStream_t s) {
+                            hipStream_t s) {
     if (!n) return;
     const unsigned nt = (unsigned)(n / DEP_TILE);
+    const int ntb = (int)(n / b.nbox / DEP_TILE);
+    const int nblk = (ng / OB) * (ng / OB) * (ng / OB);
     if (mass < 128.0f)
         k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                      b.tile_total, b.nsub, (int4*)b.boxes);
+                                                      b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
     else
         k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
                                                        (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                       b.tile_total, b.nsub, (int4*)b.boxes);
+                                                       b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
 }
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
-    k_deposit_owner<<<nb * nb * nb, OWN_NT, 0, s>>>(b.arena, (const uint4*)b.slots, b.count, b.nslots, ng,
-                                                    (unsigned long long*)rho);
+    const int nblk = (int)(nb * nb * nb);
+    k_deposit_owner<<<nb * nb * nb * b.nbox, OWN_NT, 0, s>>>(b.arena, (const uint4*)b.slots, b.count, b.nslots, ng,
+                                                             (unsigned long long*)rho, nblk,
+                                                             (int64_t)ng * ng * ng);
 }
 void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
     const int64_t nt = n / DEP_TILE;
+    const int ntb = (int)(nt / b.nbox);
+    const int64_t gc = (int64_t)ng * ng * ng;
     const unsigned grid = (unsigned)(nt < 512 ? nt : 512);
     if (mass < 128.0f)
         k_deposit_deferred<true><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho,
-                                                         b.ctl, b.deferred, b.tile_total);
+                                                         b.ctl, b.deferred, b.tile_total, ntb, gc);
     else
         k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass,
-                                                          (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total);
-    k_deposit_total<<<1, 1024, 0, s>>>(b.tile_total, nt, b.ctl);
+                                                          (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total,
+                                                          ntb, gc);
+    k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
 }
 template <int NT, int GM>
 void gather_launch(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz, double deltaT,
-                   double fscal, int do_drift, float pf, const void* boxes, hipStream_t s) {
+                   double fscal, int do_drift, float pf, const void* boxes, int nbox, hipStream_t s) {
     const unsigned nb = (unsigned)(n / (NT * 4));
+    const int64_t npb = n / nbox, gc = (int64_t)ng * ng * ng;
     if (boxes)
         k_gather_slab<NT, 4, GM, true><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                         do_drift, pf, (const int4*)boxes);
+                                                         do_drift, pf, (const int4*)boxes, npb, gc);
     else
         k_gather_slab<NT, 4, GM, false><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                          do_drift, pf, nullptr);
+                                                          do_drift, pf, nullptr, npb, gc);
 }
 void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
                         double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
-                        hipStream_t s) {
+                        int nbox, hipStream_t s) {
     if (!n) return;
-    if (variant == 1) gather_launch<256, 4160>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, s);
-    else gather_launch<512, 6656>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, s);
+    if (variant == 1)
+        gather_launch<256, 4160>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, nbox, s);
+    else
+        gather_launch<512, 6656>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, nbox, s);
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {

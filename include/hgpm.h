@@ -250,6 +250,32 @@ int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
    Clobbers the density and force grids. */
 int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
 
+/* ------------------------------------------------------------- ensembles ---- */
+/* No reference counterpart (HACC_GPM runs one box per process): many boxes stepped
+   together.  Boxes whose dynamics agree (NG, NP, RL, Z_IN, Z_FIN, N_STEPS, cosmology
+   other than SS8 and NS, PK_BINS, PK_FOLDS) form one batch that every kernel of a step
+   covers in one launch; batches run on their own streams.  Seeds and spectra are per
+   box.  FP32 engine only. */
+typedef struct hgpm_ensemble hgpm_ensemble;
+int hgpm_ensemble_create(int n_boxes, const hgpm_params* params, hgpm_ensemble** out);
+int hgpm_ensemble_destroy(hgpm_ensemble* e);
+const char* hgpm_ensemble_last_error(const hgpm_ensemble* e);
+/* Number of batches formed; batch_of_box (n_boxes entries, may be NULL) maps boxes. */
+int hgpm_ensemble_batches(const hgpm_ensemble* e, int* n_batches, int* batch_of_box);
+int hgpm_ensemble_device_bytes(const hgpm_ensemble* e, size_t* bytes);
+/* Per box, as hgpm_generate_ic / hgpm_set_particles / hgpm_get_particles /
+   hgpm_power_spectrum on a single-box context. */
+int hgpm_ensemble_generate_ic(hgpm_ensemble* e, int box, const double* ipk, int ipk_bins, double ipk_k_delta,
+                              double ipk_k_max, double delta, double dot_delta, double fscal, double deltaT);
+int hgpm_ensemble_set_particles(hgpm_ensemble* e, int box, const float* pos, const float* vel);
+int hgpm_ensemble_get_particles(hgpm_ensemble* e, int box, float* pos, float* vel);
+int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k, double* pk, int64_t* count);
+/* One step of every box (hgpm_step); coeffs[b] for box b, equal within a batch. */
+int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs);
+int hgpm_ensemble_synchronize(hgpm_ensemble* e);
+/* hgpm_autotune on every batch. */
+int hgpm_ensemble_autotune(hgpm_ensemble* e, int reps);
+
 #ifdef __cplusplus
 }
 #endif

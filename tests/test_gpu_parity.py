@@ -450,3 +450,59 @@ def test_driver_end_to_end(hg, tmp_path):
     assert snap.size == 6 * 32 ** 3
     pos = snap[: 3 * 32 ** 3]
     assert pos.min() >= 0 and pos.max() < 32
+
+
+# -------------------------------------------------------------- ensemble ----
+def _ic_inputs(hg, p):
+    import tempfile
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "eh.ipk")
+        hg.write_ipk_eh(p, fn, 100.0, 20001)
+        hdr, vals = hg.read_ipk(fn)
+    a = 1.0 / (1.0 + p.z_ini)
+    delta, _ = hg.growth(p, a)
+    _, dot_delta = hg.growth(p, a - ts.deltaT / 2.0)
+    return hdr, vals, delta, dot_delta, ts
+
+
+def test_ensemble_matches_single_boxes(hg):
+    """Boxes stepped as a batch (one launch per kernel for the batch) and as separate
+    single-box contexts: bit-identical particles and P(k)."""
+    case("ensemble 2 x 32^3 (batched) + 64^3")
+    ps = [hg.make_params(32, 32, 64.0, 30.0, 0.0, 40, seed=11, ss8=0.80),
+          hg.make_params(32, 32, 64.0, 30.0, 0.0, 40, seed=12, ss8=0.75),
+          hg.make_params(64, 64, 128.0, 30.0, 0.0, 40, seed=13, ss8=0.85)]
+    n_steps = 5
+    with hg.Ensemble(ps) as e:
+        nb, of = e.batches()
+        assert nb == 2 and of[0] == of[1] != of[2]
+        singles = [hg.MemoryManager(p) for p in ps]
+        try:
+            scheds = []
+            for b, p in enumerate(ps):
+                hdr, vals, delta, dot_delta, ts = _ic_inputs(hg, p)
+                e.generate_ic(b, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+                singles[b].generate_ic(hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+                scheds.append(hg.schedule(p, n_steps, merge=True))
+            for s in range(n_steps):
+                e.step([scheds[b][s] for b in range(len(ps))])
+                for b in range(len(ps)):
+                    singles[b].step(scheds[b][s])
+            for b in range(len(ps)):
+                pe, ve = e.get_particles(b)
+                p1, v1 = singles[b].get_particles()
+                assert np.array_equal(pe, p1) and np.array_equal(ve, v1), b
+                assert np.array_equal(e.power_spectrum(b)[1], singles[b].power_spectrum(0)[1]), b
+            # boxes in one batch must share their step coefficients
+            bad = [scheds[b][0] for b in range(len(ps))]
+            c = hg.StepCoeffs()
+            c.drift0, c.kick_dt, c.kick_fscal, c.drift1, c.skip_drift0 = 0.1, 0.2, 0.3, 0.4, 0
+            bad[1] = c
+            with pytest.raises(hg.HgpmError):
+                e.step(bad)
+        finally:
+            for m in singles:
+                m.close()
