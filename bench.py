@@ -15,7 +15,7 @@ Also reported on the same line:
   fp64_plain   : the plain FP64 GPU path (reference-structured kernels in FP64) on the
                  same workload, and the FP32/FP64 speed-up;
   cpu_baseline : the reference algorithm restated on the host CPU (the oracle in the
-                 reference's FP32 rounding) on 128^3 particles x 50 steps at the same
+                 reference's FP32 rounding) on 192^3 particles x 50 steps at the same
                  resolution, threads stated.
 """
 import argparse
@@ -113,7 +113,7 @@ def run_engine(hg, p, steps, warmup, timing=True, gather_tile=-1):
     return el, tim, t_ic, dev_bytes, tune
 
 
-def cpu_baseline(hg, threads, ng=128):
+def cpu_baseline(hg, threads, ng=192):
     """Reference algorithm on the host: the oracle in the reference's FP32 rounding
     (float3 positions, FP32 CIC, reference calcGreens), on a bounded sample of the same
     workload: ng^3 particles at the same resolution (box 2000*ng/1024 Mpc/h), z = 50 -> 0
