@@ -51,7 +51,7 @@ struct hgpm_ctx {
     unsigned long long* totals = nullptr; // deposited mass per box
     unsigned long long* total = nullptr;  // = totals (of the viewed box, see BoxView)
     // deposit arena route (k_particles.hip): window arena, per-owner-block chunk slots
-    unsigned long long* arena = nullptr;
+    unsigned* arena = nullptr; // 32-bit window cells
     unsigned long long arena_alloc = 0, arena_cap = 0; // cells allocated / usable
     void* dslots = nullptr;
     unsigned* dcount = nullptr;
@@ -441,14 +441,14 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
             return bail(rc);
         c->total = c->totals;
         {
-            // window arena: 4 cells per grid cell holds every tile window of the runs
-            // measured here through z = 0 (the windows' total grows from ~1.4 G to ~4 G
-            // cells as structure forms); tiles beyond it take the atomic route
+            // window arena: 6 cells (32-bit) per grid cell holds every tile window of the
+            // runs measured here through z = 0 (the windows' total grows from ~1.4 G to
+            // ~4 G cells as structure forms); tiles beyond it take the atomic route
             const size_t nblk = (size_t)(p->ng / 16) * (p->ng / 16) * (p->ng / 16) * nbox;
             const size_t ntiles = N * nbox / (16 * 16 * 16);
-            c->arena_alloc = 4ull * G * nbox;
+            c->arena_alloc = 6ull * G * nbox;
             c->arena_cap = c->arena_alloc;
-            if ((rc = dalloc(c, &c->arena, sizeof(unsigned long long) * c->arena_alloc, "deposit arena")) ||
+            if ((rc = dalloc(c, &c->arena, sizeof(unsigned) * c->arena_alloc, "deposit arena")) ||
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
                 (rc = dalloc(c, &c->dcount, sizeof(unsigned) * nblk, "deposit counts")) ||
                 (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||

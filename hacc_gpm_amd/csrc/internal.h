@@ -54,7 +54,8 @@ void launch_zero_i64(long long* p, size_t n, hipStream_t s);
 void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 void launch_drift(PosSplitH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
 // Deposit buffers.  np3 is a multiple of HGPM_TL^3 and ng of 16 (supported sizes).
-//   arena    : cap int64 cells for the tiles' windows;
+//   arena    : cap 32-bit cells for the tiles' windows (a tile with a cell over 2^32 - 1
+//              mass units takes the atomic route);
 //   slots    : HGPM_DEP_SLOTS chunk descriptors (16 B) per 16^3 owner block;
 //   count    : chunks listed per owner block, zero between deposits;
 //   deferred : one entry per tile;
@@ -65,7 +66,7 @@ void launch_drift(PosSplitH* pos, const float* vel, int64_t np3, int ng, float p
 #define HGPM_DEP_NSUB 64
 #define HGPM_DEP_CTL (4 + HGPM_DEP_NSUB)
 struct DepositBuffers {
-    unsigned long long* arena;
+    unsigned* arena; // 32-bit window cells
     unsigned long long cap;
     void* slots;
     unsigned* count;

@@ -182,13 +182,16 @@ __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int
 
 // Accumulates the tile's CIC weights slab by slab in `win` and hands each finished
 // slab to the flush: ARENA stores every cell of the slab to arena[base + window index]
-// (window order x, y, z with extents e), otherwise the non-zero cells are added to the
-// grid with int64 atomics.  Returns this thread's share of the deposited mass.
+// (window order x, y, z with extents e) as 32-bit values, setting *over if a cell does
+// not fit (the caller then sends the tile to the atomic route; the remaining slabs are
+// skipped); otherwise the non-zero cells are added to the grid rho with int64 atomics.
+// Returns this thread's share of the deposited mass.
 template <bool ARENA, bool SMALL>
 __device__ __forceinline__ long long dep_window(unsigned long long* win, const DepRegs& R, const int lo[3],
                                                 const int e[3], const PosSplit& r0, int64_t p0, int64_t n, int ng,
-                                                float mass, unsigned long long* __restrict dst,
-                                                unsigned long long base) {
+                                                float mass, unsigned long long* __restrict rho,
+                                                unsigned* __restrict arena, unsigned long long base,
+                                                int* over) {
     const int tid = threadIdx.x;
     const int ty = min(e[1], WMAX / e[2]);
     const int tx = min(e[0], WMAX / (ty * e[2]));
@@ -244,13 +247,14 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                 for (int i = tid; i < vol; i += DEP_NT) {
                     const unsigned long long v = win[i];
                     if (ARENA) {
-                        dst[ai] = v;
+                        if (v >> 32) *over = 1;
+                        arena[ai] = (unsigned)v;
                     } else if (v) {
                         int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
                         gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
                         gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
                         gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                        atomicAdd(&dst[gidx(ng, gx, gy, gz)], v);
+                        atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
                     }
                     iz += sz;
                     int cy = 0;
@@ -263,6 +267,7 @@ __device__ __forceinline__ long long dep_window(unsigned long long* win, const D
                 }
             }
             __syncthreads();
+            if (ARENA && *over) return local; // uniform: read after the barrier
         }
     }
     return local;
@@ -309,7 +314,7 @@ __device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3]
 // let their window sizes average out.
 template <bool SMALL>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __restrict pos, int64_t n, int ng,
-                                                            float mass, unsigned long long* __restrict arena,
+                                                            float mass, unsigned* __restrict arena,
                                                             unsigned long long cap, uint4* __restrict slots,
                                                             unsigned* __restrict count, int nslots,
                                                             unsigned* __restrict deferred,
@@ -321,6 +326,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     __shared__ long long s_tot[DEP_NT / 64];
     __shared__ unsigned long long s_base;
     __shared__ int s_state; // 0 arena route, 1 deferred (nothing reserved), 2 deferred (slots voided)
+    __shared__ int s_over;  // a window cell did not fit the 32-bit arena
     __shared__ unsigned short s_slot[DEP_MAXCH];
     const int tid = threadIdx.x;
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -347,6 +353,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     const int nch = nc[0] * nc[1] * nc[2];
     const unsigned long long V = (unsigned long long)e[0] * e[1] * e[2];
     if (tid == 0) {
+        s_over = 0;
         int st = 1;
         if (nch <= DEP_MAXCH) {
             const unsigned long long sub_cap = cap / nsub;
@@ -393,13 +400,27 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
         }
         return;
     }
-    const long long local = dep_window<true, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, arena, s_base);
+    const long long local =
+        dep_window<true, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, nullptr, arena, s_base, &s_over);
+    if (s_over) {
+        // a cell over 2^32 - 1 mass units: void this tile's chunks, atomic route instead
+        for (int c = tid; c < nch; c += DEP_NT)
+            if (s_slot[c] != 0xFFFFu) {
+                const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
+                slots[(size_t)(blk0 + ch.blk) * nslots + s_slot[c]] = make_uint4(0u, 0u, 0x80000000u, 0u);
+            }
+        if (tid == 0) {
+            deferred[atomicAdd(&ctl[1], 1ull)] = (unsigned)tile;
+            tile_total[tile] = 0ull; // written by k_deposit_deferred
+        }
+        return;
+    }
     dep_tile_total(local, s_tot, &tile_total[tile]);
 }
 
 // One workgroup per OB^3 owner block: sum the listed chunks in LDS, write every cell.
 // Resets the block's chunk count for the next deposit.
-__global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long long* __restrict arena,
+__global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __restrict arena,
                                                           const uint4* __restrict slots,
                                                           unsigned* __restrict count, int nslots, int ng,
                                                           unsigned long long* __restrict rho, int nblk,
@@ -448,6 +469,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned long lo
                 const bool in = i < vol;
                 at[u] = in ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
                 const unsigned long long a = arena[off + (in ? ((uint64_t)lx * e1 + ly) * e2 + lz : 0ull)];
+                // 32-bit arena cells: tiles whose cells exceed them took the atomic route
                 v[u] = in ? a : 0ull;
             }
 #pragma unroll
@@ -483,7 +505,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
         int lo[3], e[3];
         dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
         unsigned long long* box_rho = rho + (int64_t)(tile / ntb) * gcells;
-        const long long local = dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, box_rho, 0ull);
+        const long long local =
+            dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, box_rho, nullptr, 0ull, nullptr);
         dep_tile_total(local, s_tot, &tile_total[tile]);
         __syncthreads();
     }

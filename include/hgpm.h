@@ -236,7 +236,7 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    3 = 16x512, 4 = 8x1024, 5 = 16x1024, where the tile fits LDS); "fft_variants" (get
    only) is the number of variant ids (HGPM_FFT_VARIANTS) and "fft_variant_mask" (get
    only) has bit v set when variant v is available for this grid.  Deposit routing (FP32): "deposit_arena" = usable cells of the window
-   arena (< 0 restores the full allocation of 4 cells per grid cell), "deposit_slots"
+   arena (< 0 restores the full allocation of 6 32-bit cells per grid cell), "deposit_slots"
    = chunk descriptors per 16^3 owner block (1..64); tiles that do not fit take the
    atomic route, with identical results.  "deposit_deferred" (get only, synchronises)
    = tiles that took the atomic route in the last deposit.  Unknown keys return

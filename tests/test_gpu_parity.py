@@ -89,6 +89,25 @@ def test_deposit_routes(hg, orc, ng, np_):
         assert m.get_tunable("deposit_deferred") < ntiles
 
 
+@pytest.mark.parametrize("np_", [64, 32])
+def test_deposit_dense_cells_take_the_atomic_route(hg, orc, np_):
+    """Tiles whose window holds a cell above 2^32 - 1 mass units (hundreds of particles
+    of one tile in one cell) cannot use the 32-bit arena: they are deferred to the
+    atomic route and the density stays bit-exact."""
+    ng = 64
+    case(f"deposit dense cells ng={ng} np={np_}")
+    rng = np.random.default_rng(77 + np_)
+    with mk(hg, ng, np_) as m:
+        x = (np.array([20.0, 33.0, 41.0]) + rng.normal(0, 0.3, (m.N, 3))) % ng
+        cell = np.floor(x).astype(np.int32) % ng
+        off = (x - np.floor(x)).astype(np.float32)
+        off[off >= 1.0] = 0.0
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        assert np.array_equal(m.get_density(), orc.cic_split(ng, cell, off, (ng / np_) ** 3))
+        assert m.get_tunable("deposit_deferred") > 0
+
+
 def test_deposit_mass_ratio(hg, orc):
     case("deposit np=32 ng=64 mass 8")
     rng = np.random.default_rng(1)
