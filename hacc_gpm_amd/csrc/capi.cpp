@@ -43,10 +43,9 @@ struct hgpm_ctx {
     // FP32 engine
     PosSplitH* pos = nullptr;
     float* vel = nullptr;
-    float2* A = nullptr;             // spectrum / Fz^ (8 B per cell)
-    float2* B = nullptr;             // int64 density / Fx^+iFy^ / (Fx,Fy) (8 B per cell)
-    float* fz = nullptr;             // Fz (4 B per cell)
-    float2 *h2 = nullptr, *h3 = nullptr; // half spectra Fx^, Fy^ of the solve (A holds rho^ / Fz^)
+    float2* A = nullptr;             // spectrum (8 B per cell) / half spectra of the solve
+    float2* B = nullptr;             // int64 density (8 B per cell)
+    float* psi = nullptr;            // potential of the solve (4 B per cell); F = its central differences
     unsigned long long* ctl = nullptr;    // deposit control words (shared by the boxes)
     unsigned long long* totals = nullptr; // deposited mass per box
     unsigned long long* total = nullptr;  // = totals (of the viewed box, see BoxView)
@@ -61,7 +60,7 @@ struct hgpm_ctx {
     bool boxes_valid = false; // ... and the positions have not changed since
     int nslots = HGPM_DEP_SLOTS;
     float2 *twf = nullptr, *twr = nullptr, *twf_ic = nullptr;
-    float *sk = nullptr, *hs = nullptr;
+    float* hs = nullptr; // sin^2(pi m/ng), the Green's function's per-axis terms
     // FP64 plain path
     double *pos64 = nullptr, *vel64 = nullptr, *grad64 = nullptr;
     double2* grid64 = nullptr;
@@ -80,7 +79,7 @@ struct hgpm_ctx {
     std::vector<KStat> stats;
     std::vector<PendingEv> pending;
     std::vector<hipEvent_t> evpool;
-    // tunables: column-pass and x_solve variants of the FFT (see k_fft.hip), set by
+    // tunables: column-pass and x-pass variants of the FFT (see k_fft.hip), set by
     // hgpm_set_tunable or hgpm_autotune
     int var_cols = 2, var_xs = 2;
     int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
@@ -261,9 +260,9 @@ int f32_deposit(hgpm_ctx* c) {
     return HGPM_OK;
 }
 
-// Poisson solve and gradient on half spectra: density (B, int64) -> rho^ (A, pitch hz)
-// -> Fx^, Fy^ (h2, h3) and U = i g rho^ (A, Fz^ = sin k_z U) -> (Fx, Fy) float2 in B and
-// Fz in fz.
+// Poisson solve on half spectra: density (B, int64) -> rho^ (A, pitch hz) -> psi^ =
+// g rho^ -> potential psi (real, 4 B per cell).  The forces are the central differences
+// of psi, formed by the gather (and by hgpm_get_force).
 int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {
@@ -275,22 +274,16 @@ int f32_solve(hgpm_ctx* c) {
         f->pass_half(c->A, 1, -1, c->twf, c->var_cols, c->nbox, c->st);
     }
     {
-        Timed t(c, "fft_x_solve");
-        f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, c->var_xs, c->nbox, c->st);
+        Timed t(c, "fft_x_psi");
+        f->x_psi(c->A, c->twf, c->twr, c->hs, c->var_xs, c->nbox, c->st);
     }
     {
         Timed t(c, "fft_inv_y");
-        f->pass_half(c->h2, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
-        f->pass_half(c->h3, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
         f->pass_half(c->A, 1, +1, c->twf, c->var_cols, c->nbox, c->st);
     }
     {
-        Timed t(c, "fft_inv_z_fxy");
-        f->inv_z_c2r_xy(c->h2, c->h3, c->B, c->twf, c->nbox, c->st);
-    }
-    {
-        Timed t(c, "fft_inv_z_fz");
-        f->inv_z_c2r_pair(c->A, c->sk, c->fz, c->twf, c->nbox, c->st);
+        Timed t(c, "fft_inv_z_c2r");
+        f->inv_z_c2r_real(c->A, c->psi, c->twf, c->nbox, c->st);
     }
     LAUNCHCHK(c);
     return HGPM_OK;
@@ -433,9 +426,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
             (rc = dalloc(c, &c->vel, sizeof(float) * 3 * NB, "vel")) ||
             (rc = dalloc(c, &c->A, sizeof(float2) * GB, "grid A")) ||
             (rc = dalloc(c, &c->B, sizeof(float2) * GB, "grid B")) ||
-            (rc = dalloc(c, &c->fz, sizeof(float) * GB, "Fz")) ||
-            (rc = dalloc(c, &c->h2, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz * nbox, "half spectrum Fx")) ||
-            (rc = dalloc(c, &c->h3, sizeof(float2) * (size_t)p->ng * p->ng * c->fft->hz * nbox, "half spectrum Fy")) ||
+            (rc = dalloc(c, &c->psi, sizeof(float) * GB, "potential")) ||
             (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")) ||
             (rc = dalloc(c, &c->totals, sizeof(unsigned long long) * nbox, "deposit totals")))
             return bail(rc);
@@ -464,18 +455,14 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
         if (c->np != c->ng) {
             if ((rc = upload_twiddles<float2>(c, c->fftic, &c->twf_ic, nullptr))) return bail(rc);
         } else c->twf_ic = c->twf;
-        // per-axis solve tables: sin(2 pi m_s/ng) (signed mode) and sin^2(pi m/ng)
-        std::vector<float> sk(p->ng), hs(p->ng);
+        // per-axis Green's function table sin^2(pi m/ng)
+        std::vector<float> hs(p->ng);
         for (int m = 0; m < p->ng; m++) {
-            int ms = m > p->ng / 2 - 1 ? m - p->ng : m;
-            sk[m] = (float)sin(2.0 * HGPM_PI_D * ms / p->ng);
             double h = sin(HGPM_PI_D * m / p->ng);
             hs[m] = (float)(h * h);
         }
-        if ((rc = dalloc(c, &c->sk, sizeof(float) * p->ng, "sk")) || (rc = dalloc(c, &c->hs, sizeof(float) * p->ng, "hs")))
-            return bail(rc);
-        if (hipMemcpy(c->sk, sk.data(), sizeof(float) * p->ng, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->hs, hs.data(), sizeof(float) * p->ng, hipMemcpyHostToDevice) != hipSuccess) {
+        if ((rc = dalloc(c, &c->hs, sizeof(float) * p->ng, "hs"))) return bail(rc);
+        if (hipMemcpy(c->hs, hs.data(), sizeof(float) * p->ng, hipMemcpyHostToDevice) != hipSuccess) {
             c->err = "table upload failed";
             return bail(HGPM_ERR_HIP);
         }
@@ -523,7 +510,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) { return ctx_create(p, 1, 
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->fz, c->h2, c->h3, c->ctl, c->totals, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->sk, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->psi, c->ctl, c->totals, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -852,7 +839,7 @@ int hgpm_get_force(hgpm_ctx* c, float* f) {
     int rc = dalloc(c, &tmp, sizeof(float) * 3 * c->G, "staging");
     if (rc) return rc;
     c->bytes -= sizeof(float) * 3 * c->G;
-    launch_force_aos(c->B, c->fz, tmp, c->G, c->st);
+    launch_force_from_psi(c->psi, tmp, c->ng, c->st);
     LAUNCHCHK(c);
     HIPCHK(c, hipMemcpyAsync(f, tmp, sizeof(float) * 3 * c->G, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -889,7 +876,7 @@ int hgpm_update_velocities(hgpm_ctx* c, double deltaT, double fscal) {
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick");
-        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->B, c->fz, deltaT, fscal, 0, 0.0f, c->var_gather,
+        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->psi, deltaT, fscal, 0, 0.0f, c->var_gather,
                            c->boxes_valid ? c->boxes : nullptr, c->nbox, c->st);
     } else {
         Timed t(c, "f64_icic");
@@ -906,7 +893,7 @@ int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) 
     if (!c->have_force) return fail(c, HGPM_ERR_STATE, "no force: call hgpm_solve_gradient first");
     if (c->prec == HGPM_FP32) {
         Timed t(c, "gather_kick_drift");
-        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->B, c->fz, deltaT, fscal, 1, (float)prefactor,
+        launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->psi, deltaT, fscal, 1, (float)prefactor,
                            c->var_gather, c->boxes_valid ? c->boxes : nullptr, c->nbox, c->st);
         c->boxes_valid = false;
         LAUNCHCHK(c);
@@ -1137,7 +1124,7 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
 }
 
-// Times every column-pass and x_solve variant of this grid size on the context's own
+// Times every column-pass and x-pass variant of this grid size on the context's own
 // grids and keeps the fastest.  Overwrites the density / force grids.
 int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     if (!c) return HGPM_ERR_ARG;
@@ -1159,7 +1146,7 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
         for (int kind = 0; kind < 2; kind++) {
             auto run = [&]() {
                 if (kind == 0) f->pass_half(c->A, 1, -1, c->twf, v, c->nbox, c->st);
-                else f->x_solve_r2c(c->A, c->h2, c->h3, c->twf, c->twr, c->sk, c->hs, v, c->nbox, c->st);
+                else f->x_psi(c->A, c->twf, c->twr, c->hs, v, c->nbox, c->st);
             };
             run();
             HIPCHK(c, hipEventRecord(a, c->st));
@@ -1211,27 +1198,24 @@ struct BoxView {
     hgpm_ctx* c;
     hgpm_params p;
     PosSplitH* pos;
-    float *vel, *fz;
-    float2 *A, *B, *h2, *h3;
+    float *vel, *psi;
+    float2 *A, *B;
     unsigned long long* total;
     int nbox;
     BoxView(hgpm_ctx* c_, int b, const hgpm_params& bp) : c(c_) {
-        p = c->p; pos = c->pos; vel = c->vel; fz = c->fz; A = c->A; B = c->B; h2 = c->h2; h3 = c->h3;
+        p = c->p; pos = c->pos; vel = c->vel; psi = c->psi; A = c->A; B = c->B;
         total = c->total; nbox = c->nbox;
-        const int64_t hn = (int64_t)c->ng * c->ng * c->fft->hz;
         c->p = bp;
         c->pos += (int64_t)b * c->N;
         c->vel += 3 * (int64_t)b * c->N;
-        c->fz += (int64_t)b * c->G;
+        c->psi += (int64_t)b * c->G;
         c->A += (int64_t)b * c->G;
         c->B += (int64_t)b * c->G;
-        c->h2 += (int64_t)b * hn;
-        c->h3 += (int64_t)b * hn;
         c->total += b;
         c->nbox = 1;
     }
     ~BoxView() {
-        c->p = p; c->pos = pos; c->vel = vel; c->fz = fz; c->A = A; c->B = B; c->h2 = h2; c->h3 = h3;
+        c->p = p; c->pos = pos; c->vel = vel; c->psi = psi; c->A = A; c->B = B;
         c->total = total; c->nbox = nbox;
         c->boxes_valid = false;
     }

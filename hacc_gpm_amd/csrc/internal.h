@@ -10,7 +10,7 @@ namespace hgpm {
 struct FftOps {
     int n;
     int stages;
-    int n_variants;      // column-pass / x_solve variant ids 0..n_variants-1
+    int n_variants;      // column-pass / x-pass variant ids 0..n_variants-1
     unsigned valid_mask; // bit v set: variant v usable for this length
     void (*twiddles)(double* fwd, double* rev); // stages*n complex each
     // axis 2 = rows (z), 1 = y, 0 = x; variant selects the column-pass shape
@@ -24,13 +24,11 @@ struct FftOps {
     void (*pass_half)(float2* d, int axis, int dir, const float2* tw, int variant, int nbox, hipStream_t s);
     void (*fwd_z_r2c)(const long long* rho, const unsigned long long* totals, float2* out, const float2* tw,
                       int nbox, hipStream_t s);
-    void (*x_solve_r2c)(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr,
-                        const float* sk, const float* hs, int variant, int nbox, hipStream_t s);
-    void (*inv_z_c2r_xy)(const float2* hx, const float2* hy, float2* fxy, const float2* tw, int nbox,
-                         hipStream_t s);
-    // Fz from U = i g rho^ (half spectra after the inverse x, y passes): Fz^ = sk[k_z] U
-    void (*inv_z_c2r_pair)(const float2* hz, const float* sk, float* fz, const float2* tw, int nbox,
-                           hipStream_t s);
+    // forward x, psi^ = g rho^, inverse x, in place (the potential's half spectrum)
+    void (*x_psi)(float2* h1, const float2* twf, const float2* twr, const float* hs, int variant, int nbox,
+                  hipStream_t s);
+    // real rows from their half spectra (after the inverse x, y passes)
+    void (*inv_z_c2r_real)(const float2* hz, float* out, const float2* tw, int nbox, hipStream_t s);
 };
 
 #define HGPM_FFT_SIZES(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
@@ -86,10 +84,11 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float ma
                              const DepositBuffers& b, hipStream_t s);
 // variant 0: 512-thread workgroups on half tiles; 1: 256-thread workgroups on quarter tiles
 // boxes: the half-tile boxes of the last deposit when the positions have not changed
-// since, else nullptr (the gather reduces them itself)
-void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
-                        int nbox, hipStream_t s);
+// since, else nullptr (the gather reduces them itself); psi: the potential of the solve
+// (the forces are its central differences)
+void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float* psi, double deltaT,
+                        double fscal, int do_drift, float pf, int variant, const void* boxes, int nbox,
+                        hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
                               int ng, hipStream_t s);
@@ -100,7 +99,8 @@ void launch_to_internal(const PosSplitH* src, const float* vsrc, PosSplitH* dst,
 void launch_to_reference(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t np3, int np,
                          hipStream_t s);
 void launch_fold_split(PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
-void launch_force_aos(const float2* fxy, const float* fz, float* out, int64_t g3, hipStream_t s);
+// forces (3 floats per cell) of one box from its potential, as the gather forms them
+void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s);
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s);
 
 // ---- P(k) (k_power.hip) ----

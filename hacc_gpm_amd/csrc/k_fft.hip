@@ -8,17 +8,18 @@
 //   rows  : 1-D transforms along z; a workgroup loads RC whole rows (coalesced).
 //   cols  : 1-D transforms along y (stride PZ) or x (stride N*PZ); a workgroup loads CC
 //           adjacent z values of N rows (CC*8 contiguous bytes per row).
-// The solve of a step runs on half spectra: the density and the three force
-// components are real, so only k_z = 0..N/2 is kept (row pitch HZ = N/2 + 16, the
-// padding columns held at zero) and every pass after the first touches half the data:
+// The solve of a step runs on half spectra: the density and the potential are real, so
+// only k_z = 0..N/2 is kept (row pitch HZ = N/2 + 16, the padding columns held at zero)
+// and every pass after the first touches half the data:
 //   r2c   : pairs of real rows a, b as one complex transform of a + i b, split into
 //           the half spectra of a and b;
-//   x_solve_r2c: forward x transform, then the inverse x transforms of
-//           Fx^ = i g sin k_x rho^ and of U = i g rho^: sin k_y and sin k_z are constant
-//           along an x column, so Fy^ = sin k_y U is written at once and sin k_z is
-//           applied when the z pass reads U (two inverse transforms, not three);
-//   c2r   : (Fx, Fy) as one complex transform of the Hermitian extension of
-//           Fx^ + i Fy^; Fz rows in pairs the same way.
+//   x_psi : forward x transform, psi^ = g rho^ (the Green's function), inverse x
+//           transform, in place;
+//   c2r   : psi rows in pairs as one complex transform of the Hermitian extension of
+//           psi^_a + i psi^_b.
+// The reference's gradient, F_d^ = -i sin(k_d) G rho^ with G = -g, is F_d^ = i sin(k_d)
+// psi^: the central difference F_d[n] = (psi[n + e_d] - psi[n - e_d]) / 2 of the real
+// potential, which the force gather forms from psi (one inverse transform, not three).
 #include "fft.h"
 #include "internal.h"
 
@@ -123,20 +124,17 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __rest
     }
 }
 
-// Half-spectrum x_solve.  In: h1 = rho^ after the z and y forward passes.  Forward x
-// transform, then the inverse x transforms of F_d^ = -i sin(k_d) G rho^ =
-// i g sin(k_d) rho^ with G = (0.5/N^3)/(sum_d cos k_d - 3) = -g,
-// g = (0.25/N^3)/sum_d sin^2(k_d/2), G(0) = 0 (kspace_solve_gradient,
-// src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens, src/pm_solver/cuda/calcGreens.hpp:11-26):
-// Fx^ -> h2; U = i g rho^ -> h1 and Fy^ = sin k_y U -> h3 (Fz^ = sin k_z U is formed by
-// k_rows_c2r_pair), without the spectrum leaving registers.  sk[m] =
-// sin(2 pi m_s/N) with the signed mode m_s of get_kmodes (src/haccgpm.hpp:255-274);
-// hs[m] = sin^2(pi m/N).
+// Half-spectrum x pass of the potential.  In: h1 = rho^ after the z and y forward
+// passes.  Forward x transform, psi^ = g rho^ with g = (0.25/N^3)/sum_d sin^2(k_d/2),
+// g(0) = 0, inverse x transform, written back to h1.  The reference's Green's function
+// G = (0.5/N^3)/(sum_d cos k_d - 3) = -g and gradient -i sin(k_d) (kspace_solve_gradient,
+// src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens, src/pm_solver/cuda/calcGreens.hpp:11-26;
+// get_gradient, src/pm_solver/kernels.hpp) give F_d^ = i sin(k_d) psi^, formed in real space
+// by the gather.  hs[m] = sin^2(pi m/N).
 template <int CC, int NTC>
-__global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, float2* __restrict h2,
-                                                     float2* __restrict h3, const float2* __restrict twf,
-                                                     const float2* __restrict twr, const float* __restrict sk,
-                                                     const float* __restrict hs, float gcoef) {
+__global__ __launch_bounds__(NTC) void k_x_psi(float2* __restrict h1, const float2* __restrict twf,
+                                               const float2* __restrict twr, const float* __restrict hs,
+                                               float gcoef) {
     __shared__ float2 lds[lds_elems<N, CC>()];
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int outer = tile / (HZ / CC), zc = tile % (HZ / CC); // outer = (box, y)
@@ -148,38 +146,17 @@ __global__ __launch_bounds__(NTC) void k_x_solve_r2c(float2* __restrict h1, floa
     float2 R[LS::NB][LS::R];
     const int c0 = threadIdx.x % CC;
     const int z = zc * CC + c0; // < HZ <= N
-    const float sy = sk[y];
-    float G[LS::NB][LS::R], SX[LS::NB][LS::R];
-#pragma unroll
-    for (int b = 0; b < LS::NB; b++)
-#pragma unroll
-        for (int r = 0; r < LS::R; r++) {
-            const int w = threadIdx.x + b * NTC;
-            const int n = (SS::TOT % NTC == 0 || w < SS::TOT) ? (w / CC + r * SS::NBF) : 0;
-            const float ssum = (hs[n] + hs[y]) + hs[z];
-            G[b][r] = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
-            SX[b][r] = sk[n];
-        }
+    const float hyz = hs[y] + hs[z];
     fft_tile<float2, N, CC, NTC, -1, false, false>(
         [&](int, int, int n, int c) { return h1[base + n * NP + c]; },
         [&](int b, int r, int, int, float2 v) { R[b][r] = v; }, lds, twf);
-    // i g s rho^ = g s (-rho.y, rho.x)
     fft_tile<float2, N, CC, NTC, +1, true, false>(
-        [&](int b, int r, int, int) {
-            const float f = G[b][r] * SX[b][r];
-            return make_float2(-f * R[b][r].y, f * R[b][r].x);
+        [&](int b, int r, int n, int) {
+            const float ssum = hs[n] + hyz;
+            const float g = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
+            return make_float2(g * R[b][r].x, g * R[b][r].y);
         },
-        [&](int, int, int n, int c, float2 v) { h2[base + n * NP + c] = v; }, lds, twr);
-    fft_tile<float2, N, CC, NTC, +1, true, false>(
-        [&](int b, int r, int, int) {
-            const float f = G[b][r];
-            return make_float2(-f * R[b][r].y, f * R[b][r].x);
-        },
-        [&](int, int, int n, int c, float2 v) {
-            h1[base + n * NP + c] = v;
-            h3[base + n * NP + c] = make_float2(sy * v.x, sy * v.y);
-        },
-        lds, twr);
+        [&](int, int, int n, int c, float2 v) { h1[base + n * NP + c] = v; }, lds, twr);
 }
 
 // Element n of the Hermitian extension of a half-spectrum row: h[n] for n <= N/2,
@@ -190,50 +167,33 @@ __device__ __forceinline__ float2 herm(const float2* __restrict row, int n) {
     return make_float2(v.x, -v.y);
 }
 
-// Inverse z transform of (Fx^, Fy^) half spectra: C = Fx^ + i Fy^ over all k_z, one
-// complex transform per row; its real and imaginary parts are Fx and Fy (float2 out).
+// Inverse z transform of real rows from their half spectra, rows in pairs: the
+// Hermitian extensions of a and b as one complex transform of a + i b, whose real and
+// imaginary parts are the rows.
 template <int RC>
-__global__ __launch_bounds__(NT) void k_rows_c2r_xy(const float2* __restrict hx, const float2* __restrict hy,
-                                                    float2* __restrict fxy, const float2* __restrict tw) {
-    __shared__ float2 lds[lds_elems<N, RC>()];
-    const size_t row0 = (size_t)blockIdx.x * RC;
-    fft_tile<float2, N, RC, NT, +1, false, true>(
-        [&](int, int, int n, int c) {
-            const float2 a = herm(hx + (row0 + c) * HZ, n), b = herm(hy + (row0 + c) * HZ, n);
-            return make_float2(a.x - b.y, a.y + b.x);
-        },
-        [&](int, int, int n, int c, float2 v) { fxy[(row0 + c) * N + n] = v; }, lds, tw);
-}
-
-// Inverse z transform of Fz^ = sin k_z U from the half spectra U, rows in pairs:
-// C = Fz^_a + i Fz^_b over the Hermitian extension (sin k_z is real).
-template <int RC>
-__global__ __launch_bounds__(NT) void k_rows_c2r_pair(const float2* __restrict hz, const float* __restrict sk,
-                                                      float* __restrict fz, const float2* __restrict tw) {
+__global__ __launch_bounds__(NT) void k_rows_c2r_real(const float2* __restrict hz, float* __restrict out,
+                                                      const float2* __restrict tw) {
     constexpr int PC = RC / 2;
     __shared__ float2 lds[lds_elems<N, PC>()];
     const size_t row0 = (size_t)blockIdx.x * RC;
     fft_tile<float2, N, PC, NT, +1, false, true>(
         [&](int, int, int n, int c) {
-            const float s = sk[n <= N / 2 ? n : N - n];
-            float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
-            a.x *= s; a.y *= s;
-            b.x *= s; b.y *= s;
+            const float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
             return make_float2(a.x - b.y, a.y + b.x);
         },
         [&](int, int, int n, int c, float2 v) {
-            fz[(row0 + 2 * c) * N + n] = v.x;
-            fz[(row0 + 2 * c + 1) * N + n] = v.y;
+            out[(row0 + 2 * c) * N + n] = v.x;
+            out[(row0 + 2 * c + 1) * N + n] = v.y;
         },
         lds, tw);
 }
 
 // --------------------------------------------------------- launchers ----
-// Column-pass / x_solve variants (columns per tile, threads per workgroup):
+// Column-pass / x-pass variants (columns per tile, threads per workgroup):
 //   0: (4, 256)  1: (4, 512)  2: (8, 512)  3: (16, 512)  4: (8, 1024)  5: (16, 1024)
 // chosen per transform length by the autotuner (default 2).  More threads per tile
-// means fewer elements, and registers, per thread: the fused x-solve holds the
-// spectrum between its three transforms and needs them.
+// means fewer elements, and registers, per thread: the fused x pass holds the
+// spectrum between its two transforms.
 template <typename V, int CC, int NTC>
 void cols_launch(V* d, int axis, int dir, const V* tw, int pz, hipStream_t s, int nbox = 1) {
     unsigned nb = (unsigned)((size_t)nbox * N * (pz / CC));
@@ -300,27 +260,21 @@ void fwd_z_r2c(const long long* rho, const unsigned long long* totals, float2* o
     unsigned nb = (unsigned)((size_t)nbox * N * N / RC);
     k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, totals, out, tw);
 }
-void inv_z_c2r_xy(const float2* hx, const float2* hy, float2* fxy, const float2* tw, int nbox, hipStream_t s) {
-    unsigned nb = (unsigned)((size_t)nbox * N * N / rc32());
-    k_rows_c2r_xy<rc32()><<<nb, NT, 0, s>>>(hx, hy, fxy, tw);
-}
-void inv_z_c2r_pair(const float2* hz, const float* sk, float* fz, const float2* tw, int nbox, hipStream_t s) {
+void inv_z_c2r_real(const float2* hz, float* out, const float2* tw, int nbox, hipStream_t s) {
     constexpr int RC = 2 * rc32();
     unsigned nb = (unsigned)((size_t)nbox * N * N / RC);
-    k_rows_c2r_pair<RC><<<nb, NT, 0, s>>>(hz, sk, fz, tw);
+    k_rows_c2r_real<RC><<<nb, NT, 0, s>>>(hz, out, tw);
 }
 template <int CC, int NTC>
-void x_solve_r2c_launch(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
-                        const float* hs, int nbox, hipStream_t s) {
+void x_psi_launch(float2* h1, const float2* twf, const float2* twr, const float* hs, int nbox, hipStream_t s) {
     unsigned nb = (unsigned)((size_t)nbox * N * (HZ / CC));
     const float gcoef = (float)(0.25 / ((double)N * N * N));
-    k_x_solve_r2c<CC, NTC><<<nb, NTC, 0, s>>>(h1, h2, h3, twf, twr, sk, hs, gcoef);
+    k_x_psi<CC, NTC><<<nb, NTC, 0, s>>>(h1, twf, twr, hs, gcoef);
 }
-void x_solve_r2c(float2* h1, float2* h2, float2* h3, const float2* twf, const float2* twr, const float* sk,
-                 const float* hs, int variant, int nbox, hipStream_t s) {
+void x_psi(float2* h1, const float2* twf, const float2* twr, const float* hs, int variant, int nbox, hipStream_t s) {
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        x_solve_r2c_launch<cc_of(v), nt_of(v)>(h1, h2, h3, twf, twr, sk, hs, nbox, s);
+        x_psi_launch<cc_of(v), nt_of(v)>(h1, twf, twr, hs, nbox, s);
     });
 }
 void twiddles(double* fwd, double* rev) {
@@ -336,7 +290,7 @@ namespace hgpm {
 #define HGPM_CAT(a, b) HGPM_CAT2(a, b)
 extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
     N, Plan<N>::S, NVAR, variant_mask(), twiddles, pass_f32, pass_f64, fwd_z_density,
-    HZ, pass_half, fwd_z_r2c, x_solve_r2c, inv_z_c2r_xy, inv_z_c2r_pair,
+    HZ, pass_half, fwd_z_r2c, x_psi, inv_z_c2r_real,
 };
 } // namespace hgpm
 #endif

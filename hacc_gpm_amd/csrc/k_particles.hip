@@ -530,41 +530,45 @@ __global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long
     }
 }
 
+// Force of a cell from the potential psi: the central difference
+// F_d = (psi[c + e_d] - psi[c - e_d]) / 2, the real-space form of the reference's
+// spectral gradient -i sin(k_d) (kspace_solve_gradient, src/pm_solver/cuda/kspace_solver.cu:55-90).
+// One expression for the gather and the force grid, so both give the same bits.
+__device__ __forceinline__ float central_diff(float up, float down) { return 0.5f * (up - down); }
+
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
-// summed over the 8 corners in (x,y,z) order, then optionally the drift.  A workgroup
-// takes GT_TILE consecutive particles (part of a Lagrangian tile, a compact block of
-// the lattice).  The force cells under the bounding box of the particles' base cells
-// (plus the upper neighbours) are staged in LDS with coalesced loads, in slabs of at
-// most GMAX cells: a slab covers a range of base planes in x (and of base rows in y)
-// plus one overlapping plane (row), so each particle finds all 8 corners in the slab
-// that holds its base cell and is processed in exactly one slab.
-// Workgroup shapes (threads x particles per thread, LDS budget in cells):
-//   variant 0: 512 x 4 = half a Lagrangian tile (8 x 16 x 16), 6656 cells (78 KB, 2 per CU)
-//   variant 1: 256 x 4 = a quarter tile (4 x 16 x 16), 4160 cells (49 KB, 3 per CU)
-// The budget must hold two base rows plus their neighbours of any window,
-// 4 x (ng + 1) cells for ng <= 1024.
-constexpr int GT_STAGE = 8; // staged cells per thread per load round
+// summed over the 8 corners in (x,y,z) order, then optionally the drift.  The force of
+// each corner is the central difference of the potential psi (central_diff).  A
+// workgroup takes GT_TILE consecutive particles (part of a Lagrangian tile, a compact
+// block of the lattice).  The potential under the bounding box of the particles' base
+// cells, widened by one cell below and two above (the corners and their neighbours),
+// is staged in LDS with coalesced loads, in slabs of at most GMAX cells: a slab covers
+// a range of base planes in x (and of base rows in y) plus that halo, so each particle
+// finds all 32 values it needs in the slab that holds its base cell and is processed
+// in exactly one slab.
+// Workgroup shapes (threads x particles per thread):
+//   variant 0: 512 x 4 = half a Lagrangian tile (8 x 16 x 16)
+//   variant 1: 256 x 4 = a quarter tile (4 x 16 x 16)
+// The budget must hold one base plane and row with their halo, 4 x 4 x (ng + 3) cells,
+// for ng <= 1024.
+constexpr int GT_STAGE = 8;                // staged cells per thread per load round
+constexpr int GT_GMAX = 16 * (1024 + 3);   // 64 KB of FP32 potential
+constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
 template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
 __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
-                                                       int ng, const float2* __restrict fxy,
-                                                       const float* __restrict fz, double deltaT, double fscal,
-                                                       int do_drift, float pf, const int4* __restrict boxes,
-                                                       int64_t npb, int64_t gcells) {
+                                                       int ng, const float* __restrict psi, double deltaT,
+                                                       double fscal, int do_drift, float pf,
+                                                       const int4* __restrict boxes, int64_t npb, int64_t gcells) {
     constexpr int GT_TILE = GT_NT * GT_PPT;
     static_assert(DEP_TILE % GT_TILE == 0 && GT_TILE <= DEP_TILE / 2, "tile");
-    static_assert(GMAX >= 4 * 1025, "two rows of the widest window");
-    __shared__ float2 sxy[GMAX];
-    __shared__ float sz[GMAX];
+    static_assert(GMAX >= 16 * (1024 + GT_HALO), "one base plane and row of the widest window");
+    __shared__ float sp[GMAX];
     __shared__ int red[6][GT_NT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
-    {
-        // the force grids of this tile's box (a batch holds several boxes of the same shape)
-        const int64_t box = p0 / npb;
-        fxy += box * gcells;
-        fz += box * gcells;
-    }
+    // the potential of this tile's box (a batch holds several boxes of the same shape)
+    psi += (p0 / npb) * gcells;
     // BOXED: the deposit of these positions stored the box of each half deposit tile,
     // relative to the deposit tile's first particle; no reduction, and the staging
     // loads can start while the particle loads are in flight
@@ -614,35 +618,35 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
             int a = red[d][0], c = red[3 + d][0];
             for (int w = 1; w < GT_NT / 64; w++) { a = min(a, red[d][w]); c = max(c, red[3 + d][w]); }
             lo[d] = a;
-            b[d] = c - a + 1; // base cells per axis; the staged extent adds the upper neighbour
+            b[d] = c - a + 1; // base cells per axis
         }
     }
-    const int ez = b[2] + 1;
-    // slab shape: ty base rows (+1 staged) with (ty+1)*ez <= GMAX/2, then tx base planes
-    // (+1 staged) with (tx+1)*(ty+1)*ez <= GMAX; ez <= ng+1 <= 1025 keeps both >= 1
-    int ty = min(b[1], GMAX / (2 * ez) - 1);
+    const int ez = b[2] + GT_HALO; // <= ng + 3
+    // slab shape: ty base rows (ty + 3 staged) with 4 (ty + 3) ez <= GMAX, then tx base
+    // planes (tx + 3 staged) with (tx + 3) (ty + 3) ez <= GMAX; ez <= ng + 3 <= 1027
+    // keeps both >= 1
+    int ty = min(b[1], GMAX / (4 * ez) - GT_HALO);
     ty = (b[1] + (b[1] + ty - 1) / ty - 1) / ((b[1] + ty - 1) / ty); // even split
-    int tx = min(b[0], GMAX / ((ty + 1) * ez) - 1);
+    int tx = min(b[0], GMAX / ((ty + GT_HALO) * ez) - GT_HALO);
     tx = (b[0] + (b[0] + tx - 1) / tx - 1) / ((b[0] + tx - 1) / tx);
     for (int sx0 = 0; sx0 < b[0]; sx0 += tx) {
-        const int nxs = min(tx, b[0] - sx0) + 1;
+        const int nbx = min(tx, b[0] - sx0), nxs = nbx + GT_HALO;
         for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
-            const int nys = min(ty, b[1] - sy0) + 1;
+            const int nby = min(ty, b[1] - sy0), nys = nby + GT_HALO;
             const int vol = nxs * nys * ez;
             {
                 // GT_STAGE cells per thread per round: all their loads are issued before
                 // the first LDS store, so a round costs one memory latency, not GT_STAGE
                 const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % nys, sx = syx / nys;
                 int iz = tid % ez, iy = (tid / ez) % nys, ix = (tid / ez) / nys;
-                const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+                // staged origin: one cell below the slab's first base cell on every axis
+                const int bx0 = r0.cx + lo[0] + sx0 - 1, by0 = r0.cy + lo[1] + sy0 - 1, bz0 = r0.cz + lo[2] - 1;
                 // a slab that does not cross the periodic boundary (uniform per workgroup)
                 // advances the grid index with the walk instead of wrapping each cell
                 const bool inside = bx0 >= 0 && bx0 + nxs <= ng && by0 >= 0 && by0 + nys <= ng && bz0 >= 0 &&
                                     bz0 + ez <= ng;
                 const unsigned ung = (unsigned)ng, ung2 = ung * ung;
-                const int wx0 = bx0 < 0 ? bx0 + ng : (bx0 >= ng ? bx0 - ng : bx0);
-                const int wy0 = by0 < 0 ? by0 + ng : (by0 >= ng ? by0 - ng : by0);
-                const int wz0 = bz0 < 0 ? bz0 + ng : (bz0 >= ng ? bz0 - ng : bz0);
+                const int wx0 = wrap_cell(bx0, ng), wy0 = wrap_cell(by0, ng), wz0 = wrap_cell(bz0, ng);
                 const unsigned g0 = ((unsigned)wx0 * ung + (unsigned)wy0) * ung + (unsigned)wz0;
                 unsigned gi = ((unsigned)(bx0 + ix) * ung + (unsigned)(by0 + iy)) * ung + (unsigned)(bz0 + iz);
                 const unsigned gstep = (unsigned)sz_ + (unsigned)sy * ung + (unsigned)sx * ung2;
@@ -654,6 +658,8 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                         if (inside) {
                             g[u] = gi;
                         } else {
+                            // ez <= ng + 3 and the staged origin >= -ng/2 - 1: one
+                            // compare-and-add wraps every coordinate
                             int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
                             gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
                             gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
@@ -669,23 +675,17 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                         ix += sx + cx;
                         gi += gstep + (cy ? gcy : 0u) + (cx ? gcx : 0u);
                     }
-                    float2 a[GT_STAGE];
-                    float c[GT_STAGE];
+                    float a[GT_STAGE];
 #pragma unroll
                     for (int u = 0; u < GT_STAGE; u++) {
                         // unconditional loads (the slab's first cell past its end) keep all
                         // GT_STAGE of them in flight together
                         const unsigned gg = i0 + u * GT_NT < vol ? g[u] : g0;
-                        a[u] = fxy[gg];
-                        c[u] = fz[gg];
+                        a[u] = psi[gg];
                     }
 #pragma unroll
-                    for (int u = 0; u < GT_STAGE; u++) {
-                        if (i0 + u * GT_NT < vol) {
-                            sxy[i0 + u * GT_NT] = a[u];
-                            sz[i0 + u * GT_NT] = c[u];
-                        }
-                    }
+                    for (int u = 0; u < GT_STAGE; u++)
+                        if (i0 + u * GT_NT < vol) sp[i0 + u * GT_NT] = a[u];
                 }
             }
             __syncthreads();
@@ -701,25 +701,43 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                 asm volatile("" : "+v"(p.cx), "+v"(p.cy), "+v"(p.cz), "+v"(p.fx), "+v"(p.fy), "+v"(p.fz));
                 const int lx = wrap_rel(p.cx - r0.cx, ng) - lo[0] - sx0;
                 const int ly = wrap_rel(p.cy - r0.cy, ng) - lo[1] - sy0;
-                if (lx < 0 || lx >= nxs - 1 || ly < 0 || ly >= nys - 1) continue;
+                if (lx < 0 || lx >= nbx || ly < 0 || ly >= nby) continue;
                 const int lz = wrap_rel(p.cz - r0.cz, ng) - lo[2];
                 const float gx[2] = {1.0f - p.fx, p.fx};
                 const float gy[2] = {1.0f - p.fy, p.fy};
                 const float gz[2] = {1.0f - p.fz, p.fz};
-                const int b0 = (lx * nys + ly) * ez + lz;
+                // staged index of the base corner; strides of x and y
+                const int sxs = nys * ez;
+                const int b0 = ((lx + 1) * nys + (ly + 1)) * ez + (lz + 1);
+                // the 2x2x2 corners and the 24 neighbours their differences need
+                float q[2][2][2], xm[2][2], xp[2][2], ym[2][2], yp[2][2], zm[2][2], zp[2][2];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+#pragma unroll
+                    for (int v = 0; v < 2; v++) {
+#pragma unroll
+                        for (int w = 0; w < 2; w++) q[u][v][w] = sp[b0 + u * sxs + v * ez + w];
+                        xm[u][v] = sp[b0 - sxs + u * ez + v];     // (x-1, y+u, z+v)
+                        xp[u][v] = sp[b0 + 2 * sxs + u * ez + v]; // (x+2, y+u, z+v)
+                        ym[u][v] = sp[b0 + u * sxs - ez + v];     // (x+u, y-1, z+v)
+                        yp[u][v] = sp[b0 + u * sxs + 2 * ez + v]; // (x+u, y+2, z+v)
+                        zm[u][v] = sp[b0 + u * sxs + v * ez - 1]; // (x+u, y+v, z-1)
+                        zp[u][v] = sp[b0 + u * sxs + v * ez + 2]; // (x+u, y+v, z+2)
+                    }
                 float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
-                    const int id = b0 + ((c >> 2) ? nys * ez : 0) + (((c >> 1) & 1) ? ez : 0) + (c & 1);
-                    const float2 f2 = sxy[id];
-                    const float f1 = sz[id];
-                    float w = gx[c >> 2] * gy[(c >> 1) & 1];
-                    w = w * gz[c & 1];
+                    const int cx = c >> 2, cy = (c >> 1) & 1, cz = c & 1;
+                    const float fx = cx ? central_diff(xp[cy][cz], q[0][cy][cz]) : central_diff(q[1][cy][cz], xm[cy][cz]);
+                    const float fy = cy ? central_diff(yp[cx][cz], q[cx][0][cz]) : central_diff(q[cx][1][cz], ym[cx][cz]);
+                    const float fz = cz ? central_diff(zp[cx][cy], q[cx][cy][0]) : central_diff(q[cx][cy][1], zm[cx][cy]);
+                    float w = gx[cx] * gy[cy];
+                    w = w * gz[cz];
                     const float mul = (float)(((double)w * deltaT) * fscal);
                     float t;
-                    t = mul * f2.x; dvx = dvx + t;
-                    t = mul * f2.y; dvy = dvy + t;
-                    t = mul * f1; dvz = dvz + t;
+                    t = mul * fx; dvx = dvx + t;
+                    t = mul * fy; dvy = dvy + t;
+                    t = mul * fz; dvz = dvz + t;
                 }
                 const float vx = V[k][0] + dvx, vy = V[k][1] + dvy, vz = V[k][2] + dvz;
                 vel[3 * j] = vx;
@@ -806,14 +824,18 @@ __global__ void k_fold_split(PosSplit* __restrict pos, int64_t n, int ng) {
     pos[i] = p;
 }
 
-__global__ void k_force_aos(const float2* __restrict fxy, const float* __restrict fz, float* __restrict out,
-                            int64_t g3) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Force of every cell (x, y, z order, 3 floats per cell) from the potential of one box.
+__global__ void k_force_from_psi(const float* __restrict psi, float* __restrict out, int ng) {
+    const int64_t g3 = (int64_t)ng * ng * ng;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g3) return;
-    float2 a = fxy[i];
-    out[3 * i] = a.x;
-    out[3 * i + 1] = a.y;
-    out[3 * i + 2] = fz[i];
+    const int x = (int)(i / ((int64_t)ng * ng)), y = (int)((i / ng) % ng), z = (int)(i % ng);
+    const int xp = x + 1 == ng ? 0 : x + 1, xm = x == 0 ? ng - 1 : x - 1;
+    const int yp = y + 1 == ng ? 0 : y + 1, ym = y == 0 ? ng - 1 : y - 1;
+    const int zp = z + 1 == ng ? 0 : z + 1, zm = z == 0 ? ng - 1 : z - 1;
+    out[3 * i] = central_diff(psi[gidx(ng, xp, y, z)], psi[gidx(ng, xm, y, z)]);
+    out[3 * i + 1] = central_diff(psi[gidx(ng, x, yp, z)], psi[gidx(ng, x, ym, z)]);
+    out[3 * i + 2] = central_diff(psi[gidx(ng, x, y, zp)], psi[gidx(ng, x, y, zm)]);
 }
 
 __global__ void k_scan_nonfinite(const float* __restrict v, int64_t n, int* __restrict flag) {
@@ -879,26 +901,26 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass
                                                           ntb, gc);
     k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
 }
-template <int NT, int GM>
-void gather_launch(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz, double deltaT,
-                   double fscal, int do_drift, float pf, const void* boxes, int nbox, hipStream_t s) {
+template <int NT>
+void gather_launch(PosSplitH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT, double fscal,
+                   int do_drift, float pf, const void* boxes, int nbox, hipStream_t s) {
     const unsigned nb = (unsigned)(n / (NT * 4));
     const int64_t npb = n / nbox, gc = (int64_t)ng * ng * ng;
     if (boxes)
-        k_gather_slab<NT, 4, GM, true><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                         do_drift, pf, (const int4*)boxes, npb, gc);
+        k_gather_slab<NT, 4, GT_GMAX, true><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, psi, deltaT, fscal,
+                                                              do_drift, pf, (const int4*)boxes, npb, gc);
     else
-        k_gather_slab<NT, 4, GM, false><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, fxy, fz, deltaT, fscal,
-                                                          do_drift, pf, nullptr, npb, gc);
+        k_gather_slab<NT, 4, GT_GMAX, false><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, psi, deltaT, fscal,
+                                                               do_drift, pf, nullptr, npb, gc);
 }
-void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float2* fxy, const float* fz,
-                        double deltaT, double fscal, int do_drift, float pf, int variant, const void* boxes,
-                        int nbox, hipStream_t s) {
+void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT,
+                        double fscal, int do_drift, float pf, int variant, const void* boxes, int nbox,
+                        hipStream_t s) {
     if (!n) return;
     if (variant == 1)
-        gather_launch<256, 4160>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, nbox, s);
+        gather_launch<256>(pos, vel, n, ng, psi, deltaT, fscal, do_drift, pf, boxes, nbox, s);
     else
-        gather_launch<512, 6656>(pos, vel, n, ng, fxy, fz, deltaT, fscal, do_drift, pf, boxes, nbox, s);
+        gather_launch<512>(pos, vel, n, ng, psi, deltaT, fscal, do_drift, pf, boxes, nbox, s);
 }
 void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {
@@ -924,8 +946,9 @@ void launch_fold_split(PosSplitH* pos, int64_t n, int ng, hipStream_t s) {
     if (!n) return;
     k_fold_split<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, n, ng);
 }
-void launch_force_aos(const float2* fxy, const float* fz, float* out, int64_t g3, hipStream_t s) {
-    k_force_aos<<<grid_blocks(g3, BS), BS, 0, s>>>(fxy, fz, out, g3);
+void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s) {
+    const int64_t g3 = (int64_t)ng * ng * ng;
+    k_force_from_psi<<<grid_blocks(g3, BS), BS, 0, s>>>(psi, out, ng);
 }
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s) {
     unsigned nb = grid_blocks(n, BS);

@@ -188,7 +188,10 @@ int hgpm_set_density(hgpm_ctx* ctx, const int64_t* rho);
 int hgpm_set_density_f64(hgpm_ctx* ctx, const double* rho);
 /* serial::SolveGradient (src/pm_solver/solver.cpp:145-170): forward FFT, Green's
    function x central-difference gradient in k space, 3 inverse FFTs.
-   Force F_d = Re IFFT[-i sin(k_d) G(k) rho^(k)], G = (0.5/ng^3)/(sum cos k - 3). */
+   Force F_d = Re IFFT[-i sin(k_d) G(k) rho^(k)], G = (0.5/ng^3)/(sum cos k - 3).
+   FP32 contexts compute the potential psi = Re IFFT[-G rho^] (one inverse FFT) and
+   form F_d = (psi[c + e_d] - psi[c - e_d]) / 2, the same operator in real space;
+   FP64 contexts apply it in k space as the reference does. */
 int hgpm_solve_gradient(hgpm_ctx* ctx);
 /* force grid readback: 3*ng^3 floats / doubles, AoS (Fx,Fy,Fz) per cell */
 int hgpm_get_force(hgpm_ctx* ctx, float* f);

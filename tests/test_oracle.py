@@ -165,6 +165,27 @@ def test_solve_laplacian_and_net_force(orc):
     np.testing.assert_allclose(np.fft.fftn(div), R * ratio, atol=1e-9)
 
 
+def test_solve_is_the_central_difference_of_the_potential(orc):
+    """The FP32 engine transforms one field, the potential psi = IFFT(g rho^) with
+    g = (0.25/ng^3) / sum_d sin^2(k_d/2), and forms F_d = (psi[c + e_d] - psi[c - e_d]) / 2.
+    That is the oracle's spectral gradient -i sin(k_d) G rho^ (G = -g) exactly: the
+    identity holds to rounding on a random density and on an odd grid."""
+    rng = np.random.default_rng(9)
+    for ng in (12, 15):
+        rho = 1 + 0.3 * rng.standard_normal((ng, ng, ng))
+        F = orc.solve_gradient(ng, rho).reshape(ng, ng, ng, 3)
+        k = 2 * np.pi * np.fft.fftfreq(ng)
+        s2 = np.sin(k / 2) ** 2
+        S = s2[:, None, None] + s2[None, :, None] + s2[None, None, :]
+        S[0, 0, 0] = 1.0
+        g = 0.25 / S
+        g[0, 0, 0] = 0.0
+        psi = np.fft.ifftn(g * np.fft.fftn(rho)).real
+        for d in range(3):
+            fd = (np.roll(psi, -1, d) - np.roll(psi, 1, d)) / 2
+            np.testing.assert_allclose(F[..., d], fd, rtol=0, atol=1e-12 * np.abs(F).max())
+
+
 # ----------------------------------------------------- kick and drift ----
 def test_kick_uniform_force_and_linear_field(orc):
     rng = np.random.default_rng(4)
