@@ -13,6 +13,7 @@
 #include "internal.h"
 
 #include <climits>
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -77,14 +78,20 @@ __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, 
     if constexpr (SMALL) return (long long)(int)rintf(q);
     else return (long long)__float2ll_rn(q);
 }
+// Window cell type of the arena route: 32-bit when a weight fits (SMALL), else 64-bit.
+template <bool SMALL> using DepW = typename std::conditional<SMALL, unsigned, unsigned long long>::type;
+template <bool SMALL> __device__ __forceinline__ DepW<SMALL> fixed_weight_w(float dx, float dy, float dz, float mass) {
+    return (DepW<SMALL>)fixed_weight<SMALL>(dx, dy, dz, mass);
+}
 
 // ----------------------------------------------------------------------------------
 // Deposit.  A workgroup takes one Lagrangian tile (DEP_TILE consecutive particles of the
 // internal order, a compact block in space), finds the bounding box of the cells its
 // particles touch, relative to the first particle's cell (periodic-unwrapped), and
-// accumulates the 8 fixed-point CIC weights of every particle into an int64 window over
-// that box in LDS (ds_add_u64).  A box larger than WMAX cells is processed in slabs
-// (x, then y); each slab pass accumulates the corners that fall inside the slab.
+// accumulates the 8 fixed-point CIC weights of every particle into a window over that
+// box in LDS (ds_add_u32 when a weight fits 32 bits, else ds_add_u64).  A box larger
+// than the window budget is processed in slabs (x, then y); each slab pass accumulates
+// the corners that fall inside the slab.
 //
 // The finished window cells reach the density grid without global atomics:
 //   k_deposit_windows : stores each tile's window to an arena with plain stores and cuts
@@ -99,7 +106,7 @@ __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, 
 constexpr int DEP_NT = 512;
 constexpr int DEP_PPT = 8;
 constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
-constexpr int WMAX = 9216;                 // 72 KB of int64 per workgroup
+constexpr int DEP_LDS = 65536;             // window bytes per workgroup (two per CU)
 constexpr int OB = 16;                     // owner block edge (cells)
 constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
 constexpr int OWN_NT = 256;
@@ -180,97 +187,149 @@ __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int
     }
 }
 
-// Accumulates the tile's CIC weights slab by slab in `win` and hands each finished
-// slab to the flush: ARENA stores every cell of the slab to arena[base + window index]
-// (window order x, y, z with extents e) as 32-bit values, setting *over if a cell does
-// not fit (the caller then sends the tile to the atomic route; the remaining slabs are
-// skipped); otherwise the non-zero cells are added to the grid rho with int64 atomics.
-// Returns this thread's share of the deposited mass.
-template <bool ARENA, bool SMALL>
-__device__ __forceinline__ long long dep_window(unsigned long long* win, const DepRegs& R, const int lo[3],
-                                                const int e[3], const PosSplit& r0, int64_t p0, int64_t n, int ng,
-                                                float mass, unsigned long long* __restrict rho,
-                                                unsigned* __restrict arena, unsigned long long base,
-                                                int* over) {
-    const int tid = threadIdx.x;
-    const int ty = min(e[1], WMAX / e[2]);
-    const int tx = min(e[0], WMAX / (ty * e[2]));
-    long long local = 0;
-    for (int sx0 = 0; sx0 < e[0]; sx0 += tx) {
-        const int nx = min(tx, e[0] - sx0);
-        for (int sy0 = 0; sy0 < e[1]; sy0 += ty) {
-            const int ny = min(ty, e[1] - sy0);
-            const int vol = nx * ny * e[2];
-            for (int i = tid; i < vol; i += DEP_NT) win[i] = 0ull;
-            __syncthreads();
+// Slab walk shared by both routes: calls body(sx0, nx, sy0, ny) for each slab of at
+// most WM cells of a window with extents e (x slabs of y slabs, full z rows).
+template <int WM, typename F> __device__ __forceinline__ void dep_slabs(const int e[3], F&& body) {
+    const int ty = min(e[1], WM / e[2]);
+    const int tx = min(e[0], WM / (ty * e[2]));
+    for (int sx0 = 0; sx0 < e[0]; sx0 += tx)
+        for (int sy0 = 0; sy0 < e[1]; sy0 += ty) body(sx0, min(tx, e[0] - sx0), sy0, min(ty, e[1] - sy0));
+}
+
+// Adds the corners of the tile's particles that fall in the slab (sx0.., nx) x (sy0.., ny)
+// x (all z) to win (slab order x, y, z); returns this thread's sum of the added weights.
+// ALL: the slab is the whole window, every corner is inside (no checks).
+template <bool SMALL, bool ALL, typename W>
+__device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRegs& R, const int lo[3],
+                                                             const int e[3], int sx0, int nx, int sy0, int ny,
+                                                             float mass) {
+    unsigned long long dep = 0;
+    const int ez = e[2], sxs = ny * ez;
 #pragma unroll
-            for (int k = 0; k < DEP_PPT; k++) {
-                // the weights are slab-invariant; an opaque copy of the offsets stops the
-                // compiler from hoisting all 64 of them out of the slab loop (register spill)
-                float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
-                asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
-                const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0, bz = R.rz[k] - lo[2];
-                const float gx[2] = {1.0f - ox, ox};
-                const float gy[2] = {1.0f - oy, oy};
-                const float gz[2] = {1.0f - oz, oz};
+    for (int k = 0; k < DEP_PPT; k++) {
+        // the weights are slab-invariant; an opaque copy of the offsets stops the
+        // compiler from hoisting all 64 of them out of the slab loop (register spill)
+        float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
+        if (!ALL) asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+        const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0, bz = R.rz[k] - lo[2];
+        const float gx[2] = {1.0f - ox, ox};
+        const float gy[2] = {1.0f - oy, oy};
+        const float gz[2] = {1.0f - oz, oz};
+        const int b0 = bx * sxs + by * ez + bz;
+        // one particle's 8 weights sum to about mass * 2^24 < 2^31 when SMALL
+        DepW<SMALL> pd = 0;
 #pragma unroll
-                for (int x = 0; x < 2; x++) {
-                    const int lx = bx + x;
-                    if (lx < 0 || lx >= nx) continue;
+        for (int x = 0; x < 2; x++) {
+            if (!ALL && (bx + x < 0 || bx + x >= nx)) continue;
 #pragma unroll
-                    for (int y = 0; y < 2; y++) {
-                        const int ly = by + y;
-                        if (ly < 0 || ly >= ny) continue;
-                        unsigned long long* row = &win[(lx * ny + ly) * e[2] + bz];
-                        const long long q0 = fixed_weight<SMALL>(gx[x], gy[y], gz[0], mass);
-                        const long long q1 = fixed_weight<SMALL>(gx[x], gy[y], gz[1], mass);
-                        local += q0 + q1;
-                        atomicAdd(row, (unsigned long long)q0);
-                        atomicAdd(row + 1, (unsigned long long)q1);
-                    }
-                }
+            for (int y = 0; y < 2; y++) {
+                if (!ALL && (by + y < 0 || by + y >= ny)) continue;
+                W* row = &win[b0 + x * sxs + y * ez];
+                const DepW<SMALL> q0 = fixed_weight_w<SMALL>(gx[x], gy[y], gz[0], mass);
+                const DepW<SMALL> q1 = fixed_weight_w<SMALL>(gx[x], gy[y], gz[1], mass);
+                pd += q0 + q1;
+                atomicAdd(row, (W)q0);
+                atomicAdd(row + 1, (W)q1);
             }
-            __syncthreads();
-            // flush: thread walks cells tid, tid + DEP_NT, ... of the slab with incremental
-            // (ix, iy, iz) so no division runs inside the loop
-            {
-                const int ez = e[2];
-                const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
-                int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
-                const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
-                // arena index of (ix, iy, iz), advanced with the walk: the y carry cancels,
-                // an x carry skips the rows of the window outside this slab
-                const int64_t plane = (int64_t)e[1] * ez;
-                int64_t ai = (int64_t)base + ((int64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz;
-                const int64_t astep = sz + (int64_t)sy * ez + (int64_t)sx * plane;
-                const int64_t askip = plane - (int64_t)ny * ez;
-                for (int i = tid; i < vol; i += DEP_NT) {
-                    const unsigned long long v = win[i];
-                    if (ARENA) {
-                        if (v >> 32) *over = 1;
-                        arena[ai] = (unsigned)v;
-                    } else if (v) {
-                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
-                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                        atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
-                    }
-                    iz += sz;
-                    int cy = 0;
-                    if (iz >= ez) { iz -= ez; cy = 1; }
-                    iy += sy + cy;
-                    int cx = 0;
-                    if (iy >= ny) { iy -= ny; cx = 1; }
-                    ix += sx + cx;
-                    ai += astep + (cx ? askip : 0);
-                }
-            }
-            __syncthreads();
-            if (ARENA && *over) return local; // uniform: read after the barrier
         }
+        dep += pd;
     }
-    return local;
+    return dep;
+}
+
+// Arena route: accumulates the tile's window slab by slab in win (32-bit cells when
+// SMALL) and stores every cell to arena[base + window index] (window order x, y, z with
+// extents e) as a 32-bit value.  dep and fl receive this thread's sums of the deposited
+// weights and of the stored cells: the workgroup totals differ exactly when a cell
+// exceeded 2^32 - 1 (the caller then sends the tile to the atomic route).
+template <bool SMALL>
+__device__ __forceinline__ void dep_window_arena(DepW<SMALL>* win, const DepRegs& R, const int lo[3], const int e[3],
+                                                 float mass, unsigned* __restrict arena, unsigned long long base,
+                                                 unsigned long long& dep, unsigned long long& fl) {
+    typedef DepW<SMALL> W;
+    constexpr int WM = DEP_LDS / (int)sizeof(W);
+    const int tid = threadIdx.x;
+    const int vol = e[0] * e[1] * e[2];
+    if (vol <= WM) {
+        // the whole window in one pass; its arena copy is one contiguous range
+        for (int i = tid; i < vol; i += DEP_NT) win[i] = 0;
+        __syncthreads();
+        dep += dep_accumulate<SMALL, true>(win, R, lo, e, 0, e[0], 0, e[1], mass);
+        __syncthreads();
+        for (int i = tid; i < vol; i += DEP_NT) {
+            const W v = win[i];
+            fl += (unsigned)v;
+            arena[base + i] = (unsigned)v;
+        }
+        return;
+    }
+    dep_slabs<WM>(e, [&](int sx0, int nx, int sy0, int ny) {
+        const int ez = e[2], svol = nx * ny * ez;
+        for (int i = tid; i < svol; i += DEP_NT) win[i] = 0;
+        __syncthreads();
+        dep += dep_accumulate<SMALL, false>(win, R, lo, e, sx0, nx, sy0, ny, mass);
+        __syncthreads();
+        // thread walks cells tid, tid + DEP_NT, ... of the slab with incremental
+        // (ix, iy, iz) so no division runs inside the loop; the arena index advances with
+        // the walk (a y carry cancels, an x carry skips the rows outside this slab)
+        const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
+        int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
+        const int64_t plane = (int64_t)e[1] * ez;
+        int64_t ai = (int64_t)base + ((int64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz;
+        const int64_t astep = sz + (int64_t)sy * ez + (int64_t)sx * plane;
+        const int64_t askip = plane - (int64_t)ny * ez;
+        for (int i = tid; i < svol; i += DEP_NT) {
+            const W v = win[i];
+            fl += (unsigned)v;
+            arena[ai] = (unsigned)v;
+            iz += sz;
+            if (iz >= ez) { iz -= ez; iy += 1; }
+            iy += sy;
+            int cx = 0;
+            if (iy >= ny) { iy -= ny; cx = 1; }
+            ix += sx + cx;
+            ai += astep + (cx ? askip : 0);
+        }
+        __syncthreads();
+    });
+}
+
+// Atomic route (deferred tiles): int64 window slab by slab, non-zero cells added to the
+// density grid rho with int64 atomics.  Returns this thread's share of the deposited mass.
+template <bool SMALL>
+__device__ __forceinline__ long long dep_window_atomic(unsigned long long* win, const DepRegs& R, const int lo[3],
+                                                       const int e[3], const PosSplit& r0, int ng, float mass,
+                                                       unsigned long long* __restrict rho) {
+    constexpr int WM = DEP_LDS / 8;
+    const int tid = threadIdx.x;
+    unsigned long long dep = 0;
+    dep_slabs<WM>(e, [&](int sx0, int nx, int sy0, int ny) {
+        const int ez = e[2], svol = nx * ny * ez;
+        for (int i = tid; i < svol; i += DEP_NT) win[i] = 0ull;
+        __syncthreads();
+        dep += dep_accumulate<SMALL, false>(win, R, lo, e, sx0, nx, sy0, ny, mass);
+        __syncthreads();
+        const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
+        int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
+        const int bx0 = r0.cx + lo[0] + sx0, by0 = r0.cy + lo[1] + sy0, bz0 = r0.cz + lo[2];
+        for (int i = tid; i < svol; i += DEP_NT) {
+            const unsigned long long v = win[i];
+            if (v) {
+                int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+                atomicAdd(&rho[gidx(ng, gx, gy, gz)], v);
+            }
+            iz += sz;
+            if (iz >= ez) { iz -= ez; iy += 1; }
+            iy += sy;
+            if (iy >= ny) { iy -= ny; ix += 1; }
+            ix += sx;
+        }
+        __syncthreads();
+    });
+    return (long long)dep;
 }
 
 // Workgroup sum of the threads' deposited mass, stored by thread 0 as the tile's total
@@ -321,12 +380,14 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
                                                             unsigned long long* __restrict ctl,
                                                             unsigned long long* __restrict tile_total, int nsub,
                                                             int4* __restrict boxes, int ntb, int nblk) {
-    __shared__ unsigned long long win[WMAX];
+    __shared__ unsigned long long win_raw[DEP_LDS / 8];
+    DepW<SMALL>* win = (DepW<SMALL>*)win_raw;
     __shared__ int red[12][DEP_NT / 64];
-    __shared__ long long s_tot[DEP_NT / 64];
+    __shared__ unsigned long long s_tot[DEP_NT / 64], s_dif[DEP_NT / 64];
     __shared__ unsigned long long s_base;
     __shared__ int s_state; // 0 arena route, 1 deferred (nothing reserved), 2 deferred (slots voided)
     __shared__ int s_over;  // a window cell did not fit the 32-bit arena
+    __shared__ int s_full;  // a slot list was full (read only after the barrier that ends the listing)
     __shared__ unsigned short s_slot[DEP_MAXCH];
     const int tid = threadIdx.x;
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -354,6 +415,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     const unsigned long long V = (unsigned long long)e[0] * e[1] * e[2];
     if (tid == 0) {
         s_over = 0;
+        s_full = 0;
         int st = 1;
         if (nch <= DEP_MAXCH) {
             const unsigned long long sub_cap = cap / nsub;
@@ -367,16 +429,17 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     __syncthreads();
     // owner blocks of this tile's box (a batch holds several boxes of the same shape)
     const int blk0 = (tile / ntb) * nblk;
-    if (s_state == 0) {
+    const bool listing = s_state == 0;
+    if (listing) {
         for (int c = tid; c < nch; c += DEP_NT) {
             const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
             const unsigned s = atomicAdd(&count[blk0 + ch.blk], 1u);
             s_slot[c] = (unsigned short)(s < (unsigned)nslots ? s : 0xFFFFu);
-            if (s >= (unsigned)nslots) s_state = 2;
+            if (s >= (unsigned)nslots) s_full = 1;
         }
     }
     __syncthreads();
-    const int state = s_state;
+    const int state = listing ? (s_full ? 2 : 0) : 1;
     if (state != 1) {
         const unsigned long long base = s_base;
         for (int c = tid; c < nch; c += DEP_NT) {
@@ -400,8 +463,23 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
         }
         return;
     }
-    const long long local =
-        dep_window<true, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, nullptr, arena, s_base, &s_over);
+    unsigned long long dep = 0, fl = 0;
+    dep_window_arena<SMALL>(win, R, lo, e, mass, arena, s_base, dep, fl);
+    // workgroup totals: the stored cells sum to the deposited weights unless a cell wrapped
+    unsigned long long dif = dep - fl;
+    for (int off = 32; off > 0; off >>= 1) {
+        dep += __shfl_down(dep, off, 64);
+        dif += __shfl_down(dif, off, 64);
+    }
+    if ((tid & 63) == 0) { s_tot[tid >> 6] = dep; s_dif[tid >> 6] = dif; }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0, d = 0;
+        for (int w = 0; w < DEP_NT / 64; w++) { t += s_tot[w]; d += s_dif[w]; }
+        s_over = d != 0ull;
+        if (!d) tile_total[tile] = t;
+    }
+    __syncthreads();
     if (s_over) {
         // a cell over 2^32 - 1 mass units: void this tile's chunks, atomic route instead
         for (int c = tid; c < nch; c += DEP_NT)
@@ -413,9 +491,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
             deferred[atomicAdd(&ctl[1], 1ull)] = (unsigned)tile;
             tile_total[tile] = 0ull; // written by k_deposit_deferred
         }
-        return;
     }
-    dep_tile_total(local, s_tot, &tile_total[tile]);
 }
 
 // One workgroup per OB^3 owner block: sum the listed chunks in LDS, write every cell.
@@ -493,7 +569,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
                                                              const unsigned* __restrict deferred,
                                                              unsigned long long* __restrict tile_total, int ntb,
                                                              int64_t gcells) {
-    __shared__ unsigned long long win[WMAX];
+    __shared__ unsigned long long win[DEP_LDS / 8];
     __shared__ int red[12][DEP_NT / 64];
     __shared__ long long s_tot[DEP_NT / 64];
     const unsigned long long nd = ctl[1];
@@ -505,8 +581,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
         int lo[3], e[3];
         dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
         unsigned long long* box_rho = rho + (int64_t)(tile / ntb) * gcells;
-        const long long local =
-            dep_window<false, SMALL>(win, R, lo, e, r0, p0, n, ng, mass, box_rho, nullptr, 0ull, nullptr);
+        const long long local = dep_window_atomic<SMALL>(win, R, lo, e, r0, ng, mass, box_rho);
         dep_tile_total(local, s_tot, &tile_total[tile]);
         __syncthreads();
     }
