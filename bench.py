@@ -38,10 +38,10 @@ def kernel_bytes(name, Np, G):
     """Algorithmic bytes per launch of each timed kernel region: what the kernel must
     read and write once (no re-reads, no atomics beyond one write per cell)."""
     table = {
-        "drift": 60 * Np,                      # read pos 24 + vel 12, write pos 24
+        "drift": 44 * Np,                      # read pos 16 + vel 12, write pos 16
         "zero_density": 8 * G,                 # write int64 density
-        "deposit": 24 * Np + 8 * G,            # read pos, write density
-        "deposit_windows": 24 * Np,            # read pos (the window arena is overhead)
+        "deposit": 16 * Np + 8 * G,            # read pos (16 B packed), write density
+        "deposit_windows": 16 * Np,            # read pos (the window arena is overhead)
         "deposit_owner": 8 * G,                # write density
         # half spectra (k_z = 0..N/2) cost 4 B per grid cell
         "fft_fwd_z_r2c": 12 * G,               # read int64, write the half spectrum
@@ -49,8 +49,8 @@ def kernel_bytes(name, Np, G):
         "fft_x_psi": 8 * G,                    # read rho^, write psi^
         "fft_inv_y": 8 * G,                    # read + write one half spectrum
         "fft_inv_z_c2r": 8 * G,                # read psi^, write psi (FP32)
-        "gather_kick_drift": 72 * Np + 4 * G,  # pos+vel read and write, potential read
-        "gather_kick": 48 * Np + 4 * G,
+        "gather_kick_drift": 56 * Np + 4 * G,  # pos (16 B) + vel (12 B) read and write, potential read
+        "gather_kick": 44 * Np + 4 * G,        # pos + vel read, vel write, potential read
     }
     return table.get(name)
 
@@ -267,13 +267,14 @@ def main():
     breakdown = {k: {"ms_per_step": v[0] / args.steps, "launches": v[1],
                      "GBps": (kernel_bytes(k, Np, G) / (v[0] / v[1] * 1e-3) / 1e9) if kernel_bytes(k, Np, G) and v[1] else None}
                  for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0])}
-    # algorithmic bytes of one step of this algorithm: 96 per particle (deposit read 24,
-    # gather/kick/drift read and write 72) and 56 per grid cell (density write 8, the
-    # five half-spectrum passes 12 + 8 + 8 + 8 + 8, potential read 4).  A solve that
-    # transforms the three force fields moves 96 Np + 120 G with half spectra and
-    # 96 Np + 200 G with full complex spectra: the same step's rate against those
-    # byte counts is reported beside it.
-    step_alg = 96 * Np + 56 * G
+    # algorithmic bytes of one step of this algorithm: 72 per particle (deposit read of
+    # the 16-byte packed position, gather/kick/drift read and write of position and
+    # velocity 56) and 56 per grid cell (density write 8, the five half-spectrum passes
+    # 12 + 8 + 8 + 8 + 8, potential read 4).  With 24-byte positions and a solve that
+    # transforms the three force fields a step moves 96 Np + 120 G with half spectra and
+    # 96 Np + 200 G with full complex spectra: the same step's rate against those byte
+    # counts is reported beside it.
+    step_alg = 72 * Np + 56 * G
     step_r2c = 96 * Np + 120 * G
     step_c2c = 96 * Np + 200 * G
     out = {

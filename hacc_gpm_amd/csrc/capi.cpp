@@ -41,7 +41,7 @@ struct hgpm_ctx {
     const FftOps* fft = nullptr;    // length ng
     const FftOps* fftic = nullptr;  // length np
     // FP32 engine
-    PosSplitH* pos = nullptr;
+    PosPackH* pos = nullptr;
     float* vel = nullptr;
     float2* A = nullptr;             // spectrum (8 B per cell) / half spectra of the solve
     float2* B = nullptr;             // int64 density (8 B per cell)
@@ -385,6 +385,8 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
         return fail(nullptr, HGPM_ERR_ARG, "unsupported NG/NP " + std::to_string(p->ng) + "/" + std::to_string(p->np) +
                                                " (supported: 32 64 96 128 192 256 384 512 768 1024)");
     if (p->np > p->ng) return fail(nullptr, HGPM_ERR_UNSUPPORTED, "NP > NG is not supported");
+    if (p->ng > HGPM_MAX_CELLS)  // packed cell indices of the FP32 positions
+        return fail(nullptr, HGPM_ERR_UNSUPPORTED, "NG > " + std::to_string(HGPM_MAX_CELLS) + " is not supported");
     if (p->pk_bins < 1 || p->pk_bins > 256) return fail(nullptr, HGPM_ERR_ARG, "PK_BINS must be in [1,256]");
     if (p->pk_folds < 0 || p->pk_folds > 8) return fail(nullptr, HGPM_ERR_ARG, "PK_FOLDS must be in [0,8]");
     if (!(p->rl > 0)) return fail(nullptr, HGPM_ERR_ARG, "RL must be > 0");
@@ -422,7 +424,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
     if (c->prec == HGPM_FP32) {
         // every per-box buffer holds nbox boxes back to back
         const size_t NB = N * nbox, GB = G * nbox;
-        if ((rc = dalloc(c, &c->pos, sizeof(PosSplitH) * NB, "pos")) ||
+        if ((rc = dalloc(c, &c->pos, sizeof(PosPackH) * NB, "pos")) ||
             (rc = dalloc(c, &c->vel, sizeof(float) * 3 * NB, "vel")) ||
             (rc = dalloc(c, &c->A, sizeof(float2) * GB, "grid A")) ||
             (rc = dalloc(c, &c->B, sizeof(float2) * GB, "grid B")) ||
@@ -603,7 +605,7 @@ int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off,
     if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "split layout needs an FP32 context");
     c->boxes_valid = false;
     const size_t n = (size_t)c->N;
-    std::vector<PosSplitH> h(n);
+    std::vector<PosPackH> h(n);
     for (size_t i = 0; i < n; i++) {
         for (int d = 0; d < 3; d++) {
             int cc = cell[3 * i + d];
@@ -611,16 +613,18 @@ int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off,
             if (cc < 0 || cc >= c->ng || !(f >= 0.0f && f < 1.0f))
                 return fail(c, HGPM_ERR_ARG, "split particle out of range at index " + std::to_string(i));
         }
-        h[i] = PosSplitH{cell[3 * i], cell[3 * i + 1], cell[3 * i + 2], off[3 * i], off[3 * i + 1], off[3 * i + 2]};
+        const unsigned pc = ((unsigned)cell[3 * i] << (2 * HGPM_CELL_BITS_H)) |
+                            ((unsigned)cell[3 * i + 1] << HGPM_CELL_BITS_H) | (unsigned)cell[3 * i + 2];
+        h[i] = PosPackH{pc, off[3 * i], off[3 * i + 1], off[3 * i + 2]};
     }
-    PosSplitH* sp = nullptr;
+    PosPackH* sp = nullptr;
     float* sv = nullptr;
-    int rc = dalloc(c, &sp, sizeof(PosSplitH) * n, "staging");
+    int rc = dalloc(c, &sp, sizeof(PosPackH) * n, "staging");
     if (rc) return rc;
     rc = dalloc(c, &sv, sizeof(float) * 3 * n, "staging");
     if (rc) { (void)hipFree(sp); return rc; }
-    c->bytes -= sizeof(PosSplitH) * n + sizeof(float) * 3 * n;
-    HIPCHK(c, hipMemcpyAsync(sp, h.data(), sizeof(PosSplitH) * n, hipMemcpyHostToDevice, c->st));
+    c->bytes -= sizeof(PosPackH) * n + sizeof(float) * 3 * n;
+    HIPCHK(c, hipMemcpyAsync(sp, h.data(), sizeof(PosPackH) * n, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(sv, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->st));
     launch_to_internal(sp, sv, c->pos, c->vel, c->N, c->np, c->st);
     LAUNCHCHK(c);
@@ -638,23 +642,26 @@ int hgpm_get_particles_split(hgpm_ctx* c, int32_t* cell, float* off, float* vel)
     int rc = need_particles(c);
     if (rc) return rc;
     const size_t n = (size_t)c->N;
-    std::vector<PosSplitH> h(n);
-    PosSplitH* sp = nullptr;
+    std::vector<PosPackH> h(n);
+    PosPackH* sp = nullptr;
     float* sv = nullptr;
-    rc = dalloc(c, &sp, sizeof(PosSplitH) * n, "staging");
+    rc = dalloc(c, &sp, sizeof(PosPackH) * n, "staging");
     if (rc) return rc;
     rc = dalloc(c, &sv, sizeof(float) * 3 * n, "staging");
     if (rc) { (void)hipFree(sp); return rc; }
-    c->bytes -= sizeof(PosSplitH) * n + sizeof(float) * 3 * n;
+    c->bytes -= sizeof(PosPackH) * n + sizeof(float) * 3 * n;
     launch_to_reference(c->pos, c->vel, sp, sv, c->N, c->np, c->st);
     LAUNCHCHK(c);
-    HIPCHK(c, hipMemcpyAsync(h.data(), sp, sizeof(PosSplitH) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(h.data(), sp, sizeof(PosPackH) * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(vel, sv, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     (void)hipFree(sp);
     (void)hipFree(sv);
     for (size_t i = 0; i < n; i++) {
-        cell[3 * i] = h[i].cx; cell[3 * i + 1] = h[i].cy; cell[3 * i + 2] = h[i].cz;
+        constexpr unsigned M = (1u << HGPM_CELL_BITS_H) - 1u;
+        cell[3 * i] = (int32_t)(h[i].cell >> (2 * HGPM_CELL_BITS_H));
+        cell[3 * i + 1] = (int32_t)((h[i].cell >> HGPM_CELL_BITS_H) & M);
+        cell[3 * i + 2] = (int32_t)(h[i].cell & M);
         off[3 * i] = h[i].fx; off[3 * i + 1] = h[i].fy; off[3 * i + 2] = h[i].fz;
     }
     return HGPM_OK;
@@ -970,12 +977,12 @@ int hgpm_power_spectrum(hgpm_ctx* c, int fold, double* k, double* pk, int64_t* c
         } else {
             // fold a copy of the positions (x -> 2^fold x mod ng, exact in the split
             // layout) held in a temporary buffer
-            PosSplitH* tmp = nullptr;
-            if ((rc = dalloc(c, &tmp, sizeof(PosSplitH) * c->N, "fold copy"))) return rc;
-            c->bytes -= sizeof(PosSplitH) * c->N;
-            HIPCHK(c, hipMemcpyAsync(tmp, c->pos, sizeof(PosSplitH) * c->N, hipMemcpyDeviceToDevice, c->st));
+            PosPackH* tmp = nullptr;
+            if ((rc = dalloc(c, &tmp, sizeof(PosPackH) * c->N, "fold copy"))) return rc;
+            c->bytes -= sizeof(PosPackH) * c->N;
+            HIPCHK(c, hipMemcpyAsync(tmp, c->pos, sizeof(PosPackH) * c->N, hipMemcpyDeviceToDevice, c->st));
             for (int f = 0; f < fold; f++) launch_fold_split(tmp, c->N, c->ng, c->st);
-            PosSplitH* keep = c->pos;
+            PosPackH* keep = c->pos;
             c->pos = tmp;
             rc = f32_deposit(c);
             c->pos = keep;
@@ -1197,7 +1204,7 @@ namespace {
 struct BoxView {
     hgpm_ctx* c;
     hgpm_params p;
-    PosSplitH* pos;
+    PosPackH* pos;
     float *vel, *psi;
     float2 *A, *B;
     unsigned long long* total;

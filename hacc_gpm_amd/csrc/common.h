@@ -40,11 +40,37 @@ __device__ __forceinline__ int64_t gidx(int ng, int x, int y, int z) {
     return ((int64_t)x * ng + y) * ng + z;
 }
 
-// Particle position in the build layout: int32 cell + FP32 offset in [0,1).
-struct __align__(8) PosSplit {
+// Particle position in registers: cell index per axis + FP32 offset in [0,1).
+struct PosSplit {
     int cx, cy, cz;
     float fx, fy, fz;
 };
+// ... in memory, 16 B (one vector load or store): the cell as one 32-bit index with
+// HGPM_CELL_BITS per axis, x in the high bits (grids of up to 2^HGPM_CELL_BITS cells per
+// side), then the offsets.
+#define HGPM_CELL_BITS 10
+struct __align__(16) PosPack {
+    unsigned cell;
+    float fx, fy, fz;
+};
+__host__ __device__ __forceinline__ unsigned pack_cell(int x, int y, int z) {
+    return ((unsigned)x << (2 * HGPM_CELL_BITS)) | ((unsigned)y << HGPM_CELL_BITS) | (unsigned)z;
+}
+__host__ __device__ __forceinline__ PosSplit unpack(const PosPack& q) {
+    constexpr unsigned M = (1u << HGPM_CELL_BITS) - 1u;
+    PosSplit p;
+    p.cx = (int)(q.cell >> (2 * HGPM_CELL_BITS));
+    p.cy = (int)((q.cell >> HGPM_CELL_BITS) & M);
+    p.cz = (int)(q.cell & M);
+    p.fx = q.fx; p.fy = q.fy; p.fz = q.fz;
+    return p;
+}
+__host__ __device__ __forceinline__ PosPack pack(const PosSplit& p) {
+    PosPack q;
+    q.cell = pack_cell(p.cx, p.cy, p.cz);
+    q.fx = p.fx; q.fy = p.fy; q.fz = p.fz;
+    return q;
+}
 
 #define HGPM_WAVE 64
 

@@ -47,10 +47,14 @@ inline const FftOps* fft_ops(int n) {
 }
 
 // ---- FP32 particle kernels (k_particles.hip) ----
-struct PosSplitH { int cx, cy, cz; float fx, fy, fz; };
+// Host view of a device position (PosPack in common.h): the cell as one index with
+// HGPM_CELL_BITS bits per axis (x high), then the FP32 offsets.
+struct PosPackH { unsigned cell; float fx, fy, fz; };
+constexpr int HGPM_CELL_BITS_H = 10;
+constexpr int HGPM_MAX_CELLS = 1 << HGPM_CELL_BITS_H; // cells per side of the FP32 engine's grid
 void launch_zero_i64(long long* p, size_t n, hipStream_t s);
 void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
-void launch_drift(PosSplitH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
+void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
 // Deposit buffers.  np3 is a multiple of HGPM_TL^3 and ng of 16 (supported sizes).
 //   arena    : cap 32-bit cells for the tiles' windows (a tile with a cell over 2^32 - 1
 //              mass units takes the atomic route);
@@ -77,28 +81,28 @@ struct DepositBuffers {
     int nbox;    // boxes of the same shape stored back to back (particles, grids, owner blocks)
     unsigned long long* totals; // deposited mass per box
 };
-void launch_deposit_windows(const PosSplitH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
+void launch_deposit_windows(const PosPackH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
 void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s);
-void launch_deposit_deferred(const PosSplitH* pos, int64_t np3, int ng, float mass, long long* rho,
+void launch_deposit_deferred(const PosPackH* pos, int64_t np3, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s);
 // variant 0: 512-thread workgroups on half tiles; 1: 256-thread workgroups on quarter tiles
 // boxes: the half-tile boxes of the last deposit when the positions have not changed
 // since, else nullptr (the gather reduces them itself); psi: the potential of the solve
 // (the forces are its central differences)
-void launch_gather_kick(PosSplitH* pos, float* vel, int64_t np3, int ng, const float* psi, double deltaT,
+void launch_gather_kick(PosPackH* pos, float* vel, int64_t np3, int ng, const float* psi, double deltaT,
                         double fscal, int do_drift, float pf, int variant, const void* boxes, int nbox,
                         hipStream_t s);
 // conversions between the reference order (host arrays) and the internal tile order
-void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t np3, int np,
+void launch_split_from_float3(const float* p3, const float* v3, PosPackH* pos, float* vel, int64_t np3, int np,
                               int ng, hipStream_t s);
-void launch_split_to_float3(const PosSplitH* pos, const float* vel, float* p3, float* v3, int64_t np3, int np,
+void launch_split_to_float3(const PosPackH* pos, const float* vel, float* p3, float* v3, int64_t np3, int np,
                             hipStream_t s);
-void launch_to_internal(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t np3, int np,
+void launch_to_internal(const PosPackH* src, const float* vsrc, PosPackH* dst, float* vdst, int64_t np3, int np,
                         hipStream_t s);
-void launch_to_reference(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t np3, int np,
+void launch_to_reference(const PosPackH* src, const float* vsrc, PosPackH* dst, float* vdst, int64_t np3, int np,
                          hipStream_t s);
-void launch_fold_split(PosSplitH* pos, int64_t np3, int ng, hipStream_t s);
+void launch_fold_split(PosPackH* pos, int64_t np3, int ng, hipStream_t s);
 // forces (3 floats per cell) of one box from its potential, as the gather forms them
 void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s);
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s);
@@ -118,7 +122,7 @@ template <typename V>
 void launch_ic_scale(V* grid, int np, const double* ipk, int ipk_bins, double k_delta, double rl, hipStream_t s);
 template <typename V>
 void launch_ic_displacement(const V* amp, V* out, int np, int dim, double delta, double rl, hipStream_t s);
-void launch_ic_place_split(const float2* s_grid, PosSplitH* pos, float* vel, int np, int ng, int dim,
+void launch_ic_place_split(const float2* s_grid, PosPackH* pos, float* vel, int np, int ng, int dim,
                            double delta, float vel_mul, hipStream_t s);
 void launch_ic_place_f64(const double2* s_grid, double* pos, double* vel, int np, int ng, int dim,
                          double delta, double vel_mul, hipStream_t s);

@@ -191,7 +191,7 @@ __global__ void k_disp_f64(const double2* __restrict amp, double2* __restrict ou
 }
 
 // placeParticles for one axis: x = q ng/np + delta psi (FP64, then split), v = velMul psi.
-__global__ void k_place_split(const float2* __restrict sg, PosSplit* __restrict pos, float* __restrict vel, int np,
+__global__ void k_place_split(const float2* __restrict sg, PosPack* __restrict pos, float* __restrict vel, int np,
                               int ng, int dim, double delta, float vel_mul) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; // internal (tile) order
     const int64_t G = (int64_t)np * np * np;
@@ -207,10 +207,12 @@ __global__ void k_place_split(const float2* __restrict sg, PosSplit* __restrict 
     if (f >= 1.0f) { f = 0.0f; c += 1; }
     c %= ng;
     if (c < 0) c += ng;
-    PosSplit& p = pos[j];
+    // one axis per call: the others keep what the earlier calls placed (x, y, z order)
+    PosSplit p = dim == 0 ? PosSplit{0, 0, 0, 0.0f, 0.0f, 0.0f} : unpack(pos[j]);
     if (dim == 0) { p.cx = c; p.fx = f; }
     else if (dim == 1) { p.cy = c; p.fy = f; }
     else { p.cz = c; p.fz = f; }
+    pos[j] = pack(p);
     vel[3 * j + dim] = vel_mul * s;
 }
 __global__ void k_place_f64(const double2* __restrict sg, double* __restrict pos, double* __restrict vel, int np,
@@ -255,9 +257,9 @@ template <> void launch_ic_displacement<double2>(const double2* amp, double2* ou
                                                  double rl, hipStream_t s) {
     k_disp_f64<<<nblk((int64_t)np * np * np), BS, 0, s>>>(amp, out, np, dim, delta, rl);
 }
-void launch_ic_place_split(const float2* sg, PosSplitH* pos, float* vel, int np, int ng, int dim, double delta,
+void launch_ic_place_split(const float2* sg, PosPackH* pos, float* vel, int np, int ng, int dim, double delta,
                            float vel_mul, hipStream_t s) {
-    k_place_split<<<nblk((int64_t)np * np * np), BS, 0, s>>>(sg, (PosSplit*)pos, vel, np, ng, dim, delta, vel_mul);
+    k_place_split<<<nblk((int64_t)np * np * np), BS, 0, s>>>(sg, (PosPack*)pos, vel, np, ng, dim, delta, vel_mul);
 }
 void launch_ic_place_f64(const double2* sg, double* pos, double* vel, int np, int ng, int dim, double delta,
                          double vel_mul, hipStream_t s) {

@@ -56,13 +56,13 @@ __device__ __forceinline__ void drift_one(PosSplit& p, float vx, float vy, float
     carry_wrap(p.cz, p.fz, ng);
 }
 
-__global__ __launch_bounds__(BS) void k_drift(PosSplit* __restrict pos, const float* __restrict vel,
+__global__ __launch_bounds__(BS) void k_drift(PosPack* __restrict pos, const float* __restrict vel,
                                               int64_t n, int ng, float pf) {
     int64_t i = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    PosSplit p = pos[i];
+    PosSplit p = unpack(pos[i]);
     drift_one(p, vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], pf, ng);
-    pos[i] = p;
+    pos[i] = pack(p);
 }
 
 // CIC weight of the corner (x,y,z) in FP32: ((dx*dy)*dz)*mass with dx = f or 1-f,
@@ -110,7 +110,7 @@ constexpr int DEP_LDS = 65536;             // window bytes per workgroup (two pe
 constexpr int OB = 16;                     // owner block edge (cells)
 constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
 constexpr int OWN_NT = 256;
-constexpr int OWN_U = 8;                // arena loads in flight per lane
+constexpr int OWN_U = 16;               // arena loads in flight per lane
 constexpr int DEP_NSUB = HGPM_DEP_NSUB; // sub-arenas
 constexpr int DEP_CTL_SUB = 4;          // first sub-arena counter in ctl
 static_assert(DEP_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
@@ -138,7 +138,7 @@ struct DepRegs {
 // relative cell, e = extent + 1 (the upper CIC neighbour).  The two halves of the tile
 // (particles k < DEP_PPT/2 and k >= DEP_PPT/2 of every thread: the gather's tiles) get
 // their own boxes too: hb[h] = {lo[3], base cells per axis[3]}.
-__device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int64_t p0, int64_t n, int ng,
+__device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int64_t p0, int64_t n, int ng,
                                              const PosSplit& r0, DepRegs& R, int (*red)[DEP_NT / 64], int lo[3],
                                              int e[3], int (*hb)[6] = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -151,7 +151,7 @@ __device__ __forceinline__ void dep_load_box(const PosSplit* __restrict pos, int
     for (int k = 0; k < DEP_PPT; k++) {
         const int h = k < DEP_PPT / 2 ? 0 : 1;
         const int64_t j = p0 + tid + k * DEP_NT;
-        const PosSplit p = pos[j]; // tiles are full: n is a multiple of DEP_TILE
+        const PosSplit p = unpack(pos[j]); // tiles are full: n is a multiple of DEP_TILE
         R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
         R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
         R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
@@ -372,7 +372,7 @@ __device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3]
 // no single counter takes every tile's reservation; a few hundred tiles per sub-arena
 // let their window sizes average out.
 template <bool SMALL>
-__global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __restrict pos, int64_t n, int ng,
+__global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __restrict pos, int64_t n, int ng,
                                                             float mass, unsigned* __restrict arena,
                                                             unsigned long long cap, uint4* __restrict slots,
                                                             unsigned* __restrict count, int nslots,
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosSplit* __re
     const int tid = threadIdx.x;
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int64_t p0 = (int64_t)tile * DEP_TILE;
-    const PosSplit r0 = pos[p0];
+    const PosSplit r0 = unpack(pos[p0]);
     DepRegs R;
     int lo[3], e[3], hb[2][6];
     dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
@@ -509,16 +509,19 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
     const int box = blk / nblk, lb = blk - box * nblk;
     rho += (int64_t)box * gcells;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // every slot descriptor is loaded together with the count (the unused ones are
+    // ignored), so the two loads cost one memory latency
+    uint4 dl = make_uint4(0u, 0u, 0x80000000u, 0u);
+    if (tid < nslots) dl = slots[(size_t)blk * nslots + tid];
     if (tid == 0) {
         const unsigned c = count[blk];
         count[blk] = 0u;
         s_n = (int)min(c, (unsigned)nslots);
     }
+    if (tid < nslots) desc[tid] = dl;
     for (int i = tid; i < OB * OB * OB; i += OWN_NT) acc[i] = 0ull;
     __syncthreads();
     const int nd = s_n;
-    for (int i = tid; i < nd; i += OWN_NT) desc[i] = slots[(size_t)blk * nslots + i];
-    __syncthreads();
     for (int d = wv; d < nd; d += OWN_NT / 64) {
         const uint4 de = desc[d];
         if (de.z >> 31) continue;
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
         // i < 4096 keeps the rounding error below the 0.5/divisor margin of (i + 0.5)
         const float rmyz = 1.0f / (float)myz, rmz = 1.0f / (float)mz;
         for (int i0 = 0; i0 < vol; i0 += OWN_U * 64) {
-            unsigned long long v[OWN_U];
+            unsigned v[OWN_U];
             int at[OWN_U];
 #pragma unroll
             for (int u = 0; u < OWN_U; u++) {
@@ -544,13 +547,13 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
                 // eight of a round are in flight together
                 const bool in = i < vol;
                 at[u] = in ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
-                const unsigned long long a = arena[off + (in ? ((uint64_t)lx * e1 + ly) * e2 + lz : 0ull)];
+                const unsigned a = arena[off + (in ? ((uint64_t)lx * e1 + ly) * e2 + lz : 0ull)];
                 // 32-bit arena cells: tiles whose cells exceed them took the atomic route
-                v[u] = in ? a : 0ull;
+                v[u] = in ? a : 0u;
             }
 #pragma unroll
             for (int u = 0; u < OWN_U; u++)
-                if (v[u]) atomicAdd(&acc[at[u]], v[u]);
+                if (v[u]) atomicAdd(&acc[at[u]], (unsigned long long)v[u]);
         }
     }
     __syncthreads();
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
 
 // Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
 template <bool SMALL>
-__global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __restrict pos, int64_t n, int ng,
+__global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosPack* __restrict pos, int64_t n, int ng,
                                                              float mass, unsigned long long* __restrict rho,
                                                              const unsigned long long* __restrict ctl,
                                                              const unsigned* __restrict deferred,
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosSplit* __r
     for (unsigned long long q = blockIdx.x; q < nd; q += gridDim.x) {
         const unsigned tile = deferred[q];
         const int64_t p0 = (int64_t)tile * DEP_TILE;
-        const PosSplit r0 = pos[p0];
+        const PosSplit r0 = unpack(pos[p0]);
         DepRegs R;
         int lo[3], e[3];
         dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
@@ -631,7 +634,7 @@ constexpr int GT_GMAX = 16 * (1024 + 3);   // 64 KB of FP32 potential
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
 template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
-__global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos, float* __restrict vel, int64_t n,
+__global__ __launch_bounds__(GT_NT) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float* __restrict psi, double deltaT,
                                                        double fscal, int do_drift, float pf,
                                                        const int4* __restrict boxes, int64_t npb, int64_t gcells) {
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
     // BOXED: the deposit of these positions stored the box of each half deposit tile,
     // relative to the deposit tile's first particle; no reduction, and the staging
     // loads can start while the particle loads are in flight
-    const PosSplit r0 = pos[BOXED ? p0 / DEP_TILE * DEP_TILE : p0];
+    const PosSplit r0 = unpack(pos[BOXED ? p0 / DEP_TILE * DEP_TILE : p0]);
     PosSplit P[GT_PPT];
     float V[GT_PPT][3];
     int lo[3], b[3];
@@ -659,7 +662,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
 #pragma unroll
         for (int k = 0; k < GT_PPT; k++) {
             const int64_t j = p0 + tid + k * GT_NT;
-            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
+            P[k] = unpack(pos[j]); // tiles are full: n is a multiple of GT_TILE
             V[k][0] = vel[3 * j];
             V[k][1] = vel[3 * j + 1];
             V[k][2] = vel[3 * j + 2];
@@ -669,7 +672,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
 #pragma unroll
         for (int k = 0; k < GT_PPT; k++) {
             const int64_t j = p0 + tid + k * GT_NT;
-            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
+            P[k] = unpack(pos[j]); // tiles are full: n is a multiple of GT_TILE
             V[k][0] = vel[3 * j];
             V[k][1] = vel[3 * j + 1];
             V[k][2] = vel[3 * j + 2];
@@ -820,7 +823,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
                 vel[3 * j + 2] = vz;
                 if (do_drift) {
                     drift_one(p, vx, vy, vz, pf, ng);
-                    pos[j] = p;
+                    pos[j] = pack(p);
                 }
             }
             __syncthreads();
@@ -830,7 +833,7 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosSplit* __restrict pos,
 
 // float3 grid-unit positions in reference order -> (cell, offset) in internal order:
 // c = floor(x), f = x - c exact in FP32.
-__global__ void k_split_from_float3(const float* __restrict p3, const float* __restrict v3, PosSplit* __restrict pos,
+__global__ void k_split_from_float3(const float* __restrict p3, const float* __restrict v3, PosPack* __restrict pos,
                                     float* __restrict vel, int64_t n, int np, int ng) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -842,18 +845,18 @@ __global__ void k_split_from_float3(const float* __restrict p3, const float* __r
     carry_wrap(p.cx, p.fx, ng);
     carry_wrap(p.cy, p.fy, ng);
     carry_wrap(p.cz, p.fz, ng);
-    pos[j] = p;
+    pos[j] = pack(p);
     vel[3 * j] = v3[3 * i]; vel[3 * j + 1] = v3[3 * i + 1]; vel[3 * j + 2] = v3[3 * i + 2];
 }
 
 // internal order -> float3 grid units float((double)c + f) in reference order (snapshot layout).
-__global__ void k_split_to_float3(const PosSplit* __restrict pos, const float* __restrict vel, float* __restrict p3,
+__global__ void k_split_to_float3(const PosPack* __restrict pos, const float* __restrict vel, float* __restrict p3,
                                   float* __restrict v3, int64_t n, int np) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int qx, qy, qz;
     const int64_t i = ref_index(j, np, &qx, &qy, &qz);
-    PosSplit p = pos[j];
+    PosSplit p = unpack(pos[j]);
     p3[3 * i] = (float)((double)p.cx + (double)p.fx);
     p3[3 * i + 1] = (float)((double)p.cy + (double)p.fy);
     p3[3 * i + 2] = (float)((double)p.cz + (double)p.fz);
@@ -861,7 +864,7 @@ __global__ void k_split_to_float3(const PosSplit* __restrict pos, const float* _
 }
 
 // reference order <-> internal order for the split layout
-__global__ void k_to_internal(const PosSplit* __restrict src, const float* __restrict vsrc, PosSplit* __restrict dst,
+__global__ void k_to_internal(const PosPack* __restrict src, const float* __restrict vsrc, PosPack* __restrict dst,
                               float* __restrict vdst, int64_t n, int np) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -870,7 +873,7 @@ __global__ void k_to_internal(const PosSplit* __restrict src, const float* __res
     dst[j] = src[i];
     vdst[3 * j] = vsrc[3 * i]; vdst[3 * j + 1] = vsrc[3 * i + 1]; vdst[3 * j + 2] = vsrc[3 * i + 2];
 }
-__global__ void k_to_reference(const PosSplit* __restrict src, const float* __restrict vsrc, PosSplit* __restrict dst,
+__global__ void k_to_reference(const PosPack* __restrict src, const float* __restrict vsrc, PosPack* __restrict dst,
                                float* __restrict vdst, int64_t n, int np) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -889,14 +892,14 @@ __device__ __forceinline__ void fold1(int& c, float& f, int half) {
     c = 2 * c + (int)fl;
     f = f2 - fl;
 }
-__global__ void k_fold_split(PosSplit* __restrict pos, int64_t n, int ng) {
+__global__ void k_fold_split(PosPack* __restrict pos, int64_t n, int ng) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    PosSplit p = pos[i];
+    PosSplit p = unpack(pos[i]);
     fold1(p.cx, p.fx, ng / 2);
     fold1(p.cy, p.fy, ng / 2);
     fold1(p.cz, p.fz, ng / 2);
-    pos[i] = p;
+    pos[i] = pack(p);
 }
 
 // Force of every cell (x, y, z order, 3 floats per cell) from the potential of one box.
@@ -923,7 +926,7 @@ __global__ void k_scan_nonfinite(const float* __restrict v, int64_t n, int* __re
 } // namespace
 
 namespace hgpm {
-static_assert(sizeof(PosSplitH) == sizeof(PosSplit), "layout");
+static_assert(sizeof(PosPackH) == sizeof(PosPack), "layout");
 
 void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
     if (!n) return;
@@ -934,22 +937,22 @@ void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
 void launch_zero_i64(long long* p, size_t n, hipStream_t s) {
     launch_zero_u64((unsigned long long*)p, n, s);
 }
-void launch_drift(PosSplitH* pos, const float* vel, int64_t n, int ng, float pf, hipStream_t s) {
+void launch_drift(PosPackH* pos, const float* vel, int64_t n, int ng, float pf, hipStream_t s) {
     if (!n) return;
-    k_drift<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, vel, n, ng, pf);
+    k_drift<<<grid_blocks(n, BS), BS, 0, s>>>((PosPack*)pos, vel, n, ng, pf);
 }
-void launch_deposit_windows(const PosSplitH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
+void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s) {
     if (!n) return;
     const unsigned nt = (unsigned)(n / DEP_TILE);
     const int ntb = (int)(n / b.nbox / DEP_TILE);
     const int nblk = (ng / OB) * (ng / OB) * (ng / OB);
     if (mass < 128.0f)
-        k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
+        k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, b.arena, b.cap,
                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
                                                       b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
     else
-        k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, b.arena, b.cap,
+        k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, b.arena, b.cap,
                                                        (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
                                                        b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
 }
@@ -960,7 +963,7 @@ void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipSt
                                                              (unsigned long long*)rho, nblk,
                                                              (int64_t)ng * ng * ng);
 }
-void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass, long long* rho,
+void launch_deposit_deferred(const PosPackH* pos, int64_t n, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
     const int64_t nt = n / DEP_TILE;
@@ -968,27 +971,27 @@ void launch_deposit_deferred(const PosSplitH* pos, int64_t n, int ng, float mass
     const int64_t gc = (int64_t)ng * ng * ng;
     const unsigned grid = (unsigned)(nt < 512 ? nt : 512);
     if (mass < 128.0f)
-        k_deposit_deferred<true><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass, (unsigned long long*)rho,
+        k_deposit_deferred<true><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, (unsigned long long*)rho,
                                                          b.ctl, b.deferred, b.tile_total, ntb, gc);
     else
-        k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosSplit*)pos, n, ng, mass,
+        k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass,
                                                           (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total,
                                                           ntb, gc);
     k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
 }
 template <int NT>
-void gather_launch(PosSplitH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT, double fscal,
+void gather_launch(PosPackH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT, double fscal,
                    int do_drift, float pf, const void* boxes, int nbox, hipStream_t s) {
     const unsigned nb = (unsigned)(n / (NT * 4));
     const int64_t npb = n / nbox, gc = (int64_t)ng * ng * ng;
     if (boxes)
-        k_gather_slab<NT, 4, GT_GMAX, true><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, psi, deltaT, fscal,
+        k_gather_slab<NT, 4, GT_GMAX, true><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
                                                               do_drift, pf, (const int4*)boxes, npb, gc);
     else
-        k_gather_slab<NT, 4, GT_GMAX, false><<<nb, NT, 0, s>>>((PosSplit*)pos, vel, n, ng, psi, deltaT, fscal,
+        k_gather_slab<NT, 4, GT_GMAX, false><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
                                                                do_drift, pf, nullptr, npb, gc);
 }
-void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT,
+void launch_gather_kick(PosPackH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT,
                         double fscal, int do_drift, float pf, int variant, const void* boxes, int nbox,
                         hipStream_t s) {
     if (!n) return;
@@ -997,29 +1000,29 @@ void launch_gather_kick(PosSplitH* pos, float* vel, int64_t n, int ng, const flo
     else
         gather_launch<512>(pos, vel, n, ng, psi, deltaT, fscal, do_drift, pf, boxes, nbox, s);
 }
-void launch_split_from_float3(const float* p3, const float* v3, PosSplitH* pos, float* vel, int64_t n, int np,
+void launch_split_from_float3(const float* p3, const float* v3, PosPackH* pos, float* vel, int64_t n, int np,
                               int ng, hipStream_t s) {
     if (!n) return;
-    k_split_from_float3<<<grid_blocks(n, BS), BS, 0, s>>>(p3, v3, (PosSplit*)pos, vel, n, np, ng);
+    k_split_from_float3<<<grid_blocks(n, BS), BS, 0, s>>>(p3, v3, (PosPack*)pos, vel, n, np, ng);
 }
-void launch_split_to_float3(const PosSplitH* pos, const float* vel, float* p3, float* v3, int64_t n, int np,
+void launch_split_to_float3(const PosPackH* pos, const float* vel, float* p3, float* v3, int64_t n, int np,
                             hipStream_t s) {
     if (!n) return;
-    k_split_to_float3<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)pos, vel, p3, v3, n, np);
+    k_split_to_float3<<<grid_blocks(n, BS), BS, 0, s>>>((const PosPack*)pos, vel, p3, v3, n, np);
 }
-void launch_to_internal(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t n, int np,
+void launch_to_internal(const PosPackH* src, const float* vsrc, PosPackH* dst, float* vdst, int64_t n, int np,
                         hipStream_t s) {
     if (!n) return;
-    k_to_internal<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)src, vsrc, (PosSplit*)dst, vdst, n, np);
+    k_to_internal<<<grid_blocks(n, BS), BS, 0, s>>>((const PosPack*)src, vsrc, (PosPack*)dst, vdst, n, np);
 }
-void launch_to_reference(const PosSplitH* src, const float* vsrc, PosSplitH* dst, float* vdst, int64_t n, int np,
+void launch_to_reference(const PosPackH* src, const float* vsrc, PosPackH* dst, float* vdst, int64_t n, int np,
                          hipStream_t s) {
     if (!n) return;
-    k_to_reference<<<grid_blocks(n, BS), BS, 0, s>>>((const PosSplit*)src, vsrc, (PosSplit*)dst, vdst, n, np);
+    k_to_reference<<<grid_blocks(n, BS), BS, 0, s>>>((const PosPack*)src, vsrc, (PosPack*)dst, vdst, n, np);
 }
-void launch_fold_split(PosSplitH* pos, int64_t n, int ng, hipStream_t s) {
+void launch_fold_split(PosPackH* pos, int64_t n, int ng, hipStream_t s) {
     if (!n) return;
-    k_fold_split<<<grid_blocks(n, BS), BS, 0, s>>>((PosSplit*)pos, n, ng);
+    k_fold_split<<<grid_blocks(n, BS), BS, 0, s>>>((PosPack*)pos, n, ng);
 }
 void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s) {
     const int64_t g3 = (int64_t)ng * ng * ng;
