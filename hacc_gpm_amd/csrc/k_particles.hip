@@ -109,6 +109,7 @@ constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
 constexpr int DEP_LDS = 65536;             // window bytes per workgroup (two per CU)
 constexpr int OB = 16;                     // owner block edge (cells)
 constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
+static_assert(DEP_MAXCH <= DEP_NT, "one chunk per thread");
 constexpr int OWN_NT = 256;
 constexpr int OWN_U = 16;               // arena loads in flight per lane
 constexpr int DEP_NSUB = HGPM_DEP_NSUB; // sub-arenas
@@ -237,32 +238,44 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
     return dep;
 }
 
-// Arena route: accumulates the tile's window slab by slab in win (32-bit cells when
-// SMALL) and stores every cell to arena[base + window index] (window order x, y, z with
-// extents e) as a 32-bit value.  dep and fl receive this thread's sums of the deposited
-// weights and of the stored cells: the workgroup totals differ exactly when a cell
-// exceeded 2^32 - 1 (the caller then sends the tile to the atomic route).
+// Arena route.  The window is accumulated in win (32-bit cells when SMALL) and every
+// cell is stored to arena[base + window index] (window order x, y, z with extents e)
+// as a 32-bit value.  dep and fl receive this thread's sums of the deposited weights
+// and of the stored cells: the workgroup totals differ exactly when a cell exceeded
+// 2^32 - 1 (the caller then sends the tile to the atomic route).
+// A window of at most DEP_LDS bytes is accumulated in one pass (dep_one_accumulate,
+// then dep_one_flush: a contiguous arena range); a larger one slab by slab
+// (dep_window_arena_slabs).
+template <bool SMALL> __device__ __forceinline__ bool dep_one_pass(const int e[3]) {
+    return e[0] * e[1] * e[2] <= DEP_LDS / (int)sizeof(DepW<SMALL>);
+}
 template <bool SMALL>
-__device__ __forceinline__ void dep_window_arena(DepW<SMALL>* win, const DepRegs& R, const int lo[3], const int e[3],
-                                                 float mass, unsigned* __restrict arena, unsigned long long base,
-                                                 unsigned long long& dep, unsigned long long& fl) {
+__device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
+                                                   const int e[3], float mass, unsigned long long& dep) {
+    const int vol = e[0] * e[1] * e[2];
+    for (int i = threadIdx.x; i < vol; i += DEP_NT) win[i] = 0;
+    __syncthreads();
+    dep += dep_accumulate<SMALL, true>(win, R, lo, e, 0, e[0], 0, e[1], mass);
+}
+// after a barrier that follows dep_one_accumulate
+template <bool SMALL>
+__device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int e[3], unsigned* __restrict arena,
+                                              unsigned long long base, unsigned long long& fl) {
+    const int vol = e[0] * e[1] * e[2];
+    for (int i = threadIdx.x; i < vol; i += DEP_NT) {
+        const DepW<SMALL> v = win[i];
+        fl += (unsigned)v;
+        arena[base + i] = (unsigned)v;
+    }
+}
+template <bool SMALL>
+__device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
+                                                       const int e[3], float mass, unsigned* __restrict arena,
+                                                       unsigned long long base, unsigned long long& dep,
+                                                       unsigned long long& fl) {
     typedef DepW<SMALL> W;
     constexpr int WM = DEP_LDS / (int)sizeof(W);
     const int tid = threadIdx.x;
-    const int vol = e[0] * e[1] * e[2];
-    if (vol <= WM) {
-        // the whole window in one pass; its arena copy is one contiguous range
-        for (int i = tid; i < vol; i += DEP_NT) win[i] = 0;
-        __syncthreads();
-        dep += dep_accumulate<SMALL, true>(win, R, lo, e, 0, e[0], 0, e[1], mass);
-        __syncthreads();
-        for (int i = tid; i < vol; i += DEP_NT) {
-            const W v = win[i];
-            fl += (unsigned)v;
-            arena[base + i] = (unsigned)v;
-        }
-        return;
-    }
     dep_slabs<WM>(e, [&](int sx0, int nx, int sy0, int ny) {
         const int ez = e[2], svol = nx * ny * ez;
         for (int i = tid; i < svol; i += DEP_NT) win[i] = 0;
@@ -385,14 +398,14 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     __shared__ int red[12][DEP_NT / 64];
     __shared__ unsigned long long s_tot[DEP_NT / 64], s_dif[DEP_NT / 64];
     __shared__ unsigned long long s_base;
-    __shared__ int s_state; // 0 arena route, 1 deferred (nothing reserved), 2 deferred (slots voided)
     __shared__ int s_over;  // a window cell did not fit the 32-bit arena
-    __shared__ int s_full;  // a slot list was full (read only after the barrier that ends the listing)
+    __shared__ int s_full;  // a slot list or the sub-arena was full (read after the publishing barrier)
     __shared__ unsigned short s_slot[DEP_MAXCH];
     const int tid = threadIdx.x;
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int64_t p0 = (int64_t)tile * DEP_TILE;
     const PosSplit r0 = unpack(pos[p0]);
+    if (tid == 0) { s_over = 0; s_full = 0; } // ordered by the barrier in dep_load_box
     DepRegs R;
     int lo[3], e[3], hb[2][6];
     dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
@@ -413,33 +426,37 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     }
     const int nch = nc[0] * nc[1] * nc[2];
     const unsigned long long V = (unsigned long long)e[0] * e[1] * e[2];
-    if (tid == 0) {
-        s_over = 0;
-        s_full = 0;
-        int st = 1;
-        if (nch <= DEP_MAXCH) {
-            const unsigned long long sub_cap = cap / nsub;
-            const int sub = tile % nsub;
-            const unsigned long long b = atomicAdd(&ctl[DEP_CTL_SUB + sub], V);
-            s_base = (unsigned long long)sub * sub_cap + b;
-            st = (b + V <= sub_cap) ? 0 : 1;
-        }
-        s_state = st;
-    }
-    __syncthreads();
     // owner blocks of this tile's box (a batch holds several boxes of the same shape)
     const int blk0 = (tile / ntb) * nblk;
-    const bool listing = s_state == 0;
-    if (listing) {
-        for (int c = tid; c < nch; c += DEP_NT) {
-            const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
-            const unsigned s = atomicAdd(&count[blk0 + ch.blk], 1u);
-            s_slot[c] = (unsigned short)(s < (unsigned)nslots ? s : 0xFFFFu);
-            if (s >= (unsigned)nslots) s_full = 1;
+    // The sub-arena reservation and the chunk listing are issued first and their
+    // results published after a one-pass window is accumulated, so that their memory
+    // latency overlaps the accumulation.  A tile with more than DEP_MAXCH chunks takes
+    // the atomic route and lists nothing.
+    const bool fits = nch <= DEP_MAXCH;
+    unsigned long long rb = 0;
+    if (fits && tid == 0) rb = atomicAdd(&ctl[DEP_CTL_SUB + tile % nsub], V);
+    unsigned sl = 0xFFFFu; // DEP_MAXCH <= DEP_NT: at most one chunk per thread
+    if (fits && tid < nch) {
+        const Chunk ch = dep_chunk(tid, A, e, c0, nc, nb);
+        sl = atomicAdd(&count[blk0 + ch.blk], 1u);
+    }
+    unsigned long long dep = 0, fl = 0;
+    const bool one = dep_one_pass<SMALL>(e);
+    if (one) dep_one_accumulate<SMALL>(win, R, lo, e, mass, dep);
+    if (fits) {
+        if (tid == 0) {
+            const unsigned long long sub_cap = cap / nsub;
+            s_base = (unsigned long long)(tile % nsub) * sub_cap + rb;
+            if (rb + V > sub_cap) s_full = 1;
+        }
+        if (tid < nch) {
+            s_slot[tid] = (unsigned short)(sl < (unsigned)nslots ? sl : 0xFFFFu);
+            if (sl >= (unsigned)nslots) s_full = 1;
         }
     }
     __syncthreads();
-    const int state = listing ? (s_full ? 2 : 0) : 1;
+    // 0: arena route; 1: deferred, nothing listed; 2: deferred, listed slots voided
+    const int state = !fits ? 1 : (s_full ? 2 : 0);
     if (state != 1) {
         const unsigned long long base = s_base;
         for (int c = tid; c < nch; c += DEP_NT) {
@@ -463,8 +480,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
         }
         return;
     }
-    unsigned long long dep = 0, fl = 0;
-    dep_window_arena<SMALL>(win, R, lo, e, mass, arena, s_base, dep, fl);
+    if (one) dep_one_flush<SMALL>(win, e, arena, s_base, fl);
+    else dep_window_arena_slabs<SMALL>(win, R, lo, e, mass, arena, s_base, dep, fl);
     // workgroup totals: the stored cells sum to the deposited weights unless a cell wrapped
     unsigned long long dif = dep - fl;
     for (int off = 32; off > 0; off >>= 1) {
@@ -495,12 +512,17 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
 }
 
 // One workgroup per OB^3 owner block: sum the listed chunks in LDS, write every cell.
-// Resets the block's chunk count for the next deposit.
+// Resets the block's chunk count for the next deposit.  A wave takes one chunk at a
+// time: each lane holds up to OWN_J cells of one (y, z) plane of the chunk (p = lane +
+// 64 j, decoded once per chunk) and walks the chunk's x planes, so a cell costs an
+// add for its arena address and one for its LDS address; OWN_U planes are loaded
+// before the first LDS add so that their loads are in flight together.
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __restrict arena,
                                                           const uint4* __restrict slots,
                                                           unsigned* __restrict count, int nslots, int ng,
                                                           unsigned long long* __restrict rho, int nblk,
                                                           int64_t gcells) {
+    constexpr int OWN_J = OB * OB / 64; // plane slots per lane (a plane holds <= OB^2 cells)
     __shared__ unsigned long long acc[OB * OB * OB];
     __shared__ uint4 desc[HGPM_DEP_SLOTS];
     __shared__ int s_n;
@@ -523,45 +545,54 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
     __syncthreads();
     const int nd = s_n;
     for (int d = wv; d < nd; d += OWN_NT / 64) {
-        const uint4 de = desc[d];
+        // one descriptor per wave: its words are uniform (scalar registers)
+        uint4 de = desc[d];
+        de.x = __builtin_amdgcn_readfirstlane(de.x);
+        de.y = __builtin_amdgcn_readfirstlane(de.y);
+        de.z = __builtin_amdgcn_readfirstlane(de.z);
+        de.w = __builtin_amdgcn_readfirstlane(de.w);
         if (de.z >> 31) continue;
-        const unsigned long long off = (unsigned long long)de.x | ((unsigned long long)de.y << 32);
+        const unsigned* __restrict ch = arena + ((unsigned long long)de.x | ((unsigned long long)de.y << 32));
         const int ox = de.z & 15, oy = (de.z >> 4) & 15, oz = (de.z >> 8) & 15;
         const int mx = ((de.z >> 12) & 15) + 1, my = ((de.z >> 16) & 15) + 1, mz = ((de.z >> 20) & 15) + 1;
-        const int e1 = (int)(de.w & 0xFFFFu), e2 = (int)(de.w >> 16);
-        const int myz = my * mz, vol = mx * myz;
-        // i -> (lx, ly, lz) by multiplication with a rounded reciprocal: exact here because
-        // i < 4096 keeps the rounding error below the 0.5/divisor margin of (i + 0.5)
-        const float rmyz = 1.0f / (float)myz, rmz = 1.0f / (float)mz;
-        for (int i0 = 0; i0 < vol; i0 += OWN_U * 64) {
-            unsigned v[OWN_U];
-            int at[OWN_U];
+        const int e2 = (int)(de.w >> 16);
+        const unsigned e12 = (de.w & 0xFFFFu) * (unsigned)e2; // arena stride of an x plane
+        const int myz = my * mz;
+        // p -> (ly, lz) by multiplication with a rounded reciprocal: exact here because
+        // p < 256 keeps the rounding error below the 0.5/mz margin of (p + 0.5)
+        const float rmz = 1.0f / (float)mz;
+        for (int j = 0; j * 64 < myz; j++) {
+            const int pl = lane + 64 * j;
+            const bool pv = pl < myz;
+            const int ly = (int)(((float)pl + 0.5f) * rmz), lz = pl - ly * mz;
+            const unsigned po = pv ? (unsigned)(ly * e2 + lz) : 0u; // arena offset in the plane
+            const int la = ox * OB * OB + (oy + ly) * OB + oz + lz; // LDS cell of x plane 0
+            for (int x0 = 0; x0 < mx; x0 += OWN_U) {
+                unsigned v[OWN_U];
 #pragma unroll
-            for (int u = 0; u < OWN_U; u++) {
-                const int i = i0 + u * 64 + lane;
-                const int lx = (int)(((float)i + 0.5f) * rmyz);
-                const int r = i - lx * myz;
-                const int ly = (int)(((float)r + 0.5f) * rmz);
-                const int lz = r - ly * mz;
-                // loads are unconditional (cell 0 of the chunk past its end) so that the
-                // eight of a round are in flight together
-                const bool in = i < vol;
-                at[u] = in ? ((ox + lx) * OB + oy + ly) * OB + oz + lz : -1;
-                const unsigned a = arena[off + (in ? ((uint64_t)lx * e1 + ly) * e2 + lz : 0ull)];
-                // 32-bit arena cells: tiles whose cells exceed them took the atomic route
-                v[u] = in ? a : 0u;
+                for (int u = 0; u < OWN_U; u++) {
+                    // uniform bound; lanes past the plane load its first cell.  A chunk
+                    // lies in a window of at most DEP_MAXCH owner blocks: byte offsets
+                    // from its first cell fit 32 bits
+                    if (x0 + u < mx)
+                        v[u] = *(const unsigned*)((const char*)ch + ((unsigned)(x0 + u) * e12 + po) * 4u);
+                }
+#pragma unroll
+                for (int u = 0; u < OWN_U; u++) {
+                    // 32-bit arena cells: tiles whose cells exceed them took the atomic route
+                    if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (unsigned long long)v[u]);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < OWN_U; u++)
-                if (v[u]) atomicAdd(&acc[at[u]], (unsigned long long)v[u]);
         }
     }
     __syncthreads();
+    // thread tid writes (x, y, z) = (k, tid / OB, tid % OB) of the block for k = 0..OB-1
+    static_assert(OWN_NT == OB * OB, "one (y, z) column of the block per thread");
     const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
-    for (int i = tid; i < OB * OB * OB; i += OWN_NT) {
-        const int lx = i >> 8, ly = (i >> 4) & 15, lz = i & 15;
-        rho[gidx(ng, bx * OB + lx, by * OB + ly, bz * OB + lz)] = acc[i];
-    }
+    unsigned long long* out = rho + gidx(ng, bx * OB, by * OB + (tid >> 4), bz * OB + (tid & 15));
+    const int64_t plane = (int64_t)ng * ng;
+#pragma unroll 4
+    for (int k = 0; k < OB; k++) out[k * plane] = acc[k * OB * OB + tid];
 }
 
 // Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
@@ -633,8 +664,9 @@ constexpr int GT_STAGE = 8;                // staged cells per thread per load r
 constexpr int GT_GMAX = 16 * (1024 + 3);   // 64 KB of FP32 potential
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
+// (at least 4 waves per SIMD: two 512-thread workgroups per CU, as the LDS budget allows)
 template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
-__global__ __launch_bounds__(GT_NT) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
+__global__ __launch_bounds__(GT_NT, 4) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float* __restrict psi, double deltaT,
                                                        double fscal, int do_drift, float pf,
                                                        const int4* __restrict boxes, int64_t npb, int64_t gcells) {
@@ -713,8 +745,6 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosPack* __restrict pos, 
             const int nby = min(ty, b[1] - sy0), nys = nby + GT_HALO;
             const int vol = nxs * nys * ez;
             {
-                // GT_STAGE cells per thread per round: all their loads are issued before
-                // the first LDS store, so a round costs one memory latency, not GT_STAGE
                 const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % nys, sx = syx / nys;
                 int iz = tid % ez, iy = (tid / ez) % nys, ix = (tid / ez) / nys;
                 // staged origin: one cell below the slab's first base cell on every axis
@@ -729,41 +759,49 @@ __global__ __launch_bounds__(GT_NT) void k_gather_slab(PosPack* __restrict pos, 
                 unsigned gi = ((unsigned)(bx0 + ix) * ung + (unsigned)(by0 + iy)) * ung + (unsigned)(bz0 + iz);
                 const unsigned gstep = (unsigned)sz_ + (unsigned)sy * ung + (unsigned)sx * ung2;
                 const unsigned gcy = ung - (unsigned)ez, gcx = ung2 - (unsigned)nys * ung;
-                for (int i0 = tid; i0 < vol; i0 += GT_STAGE * GT_NT) {
-                    unsigned g[GT_STAGE];
+                // the slab's cells, GT_STAGE per thread per round: all their loads are
+                // issued before the first LDS store, so a round costs one memory latency
+                auto stage = [&](auto index) {
+                    for (int i0 = tid; i0 < vol; i0 += GT_STAGE * GT_NT) {
+                        unsigned g[GT_STAGE];
 #pragma unroll
-                    for (int u = 0; u < GT_STAGE; u++) {
-                        if (inside) {
-                            g[u] = gi;
-                        } else {
-                            // ez <= ng + 3 and the staged origin >= -ng/2 - 1: one
-                            // compare-and-add wraps every coordinate
-                            int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
-                            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-                            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                            gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                            g[u] = ((unsigned)gx * ung + (unsigned)gy) * ung + (unsigned)gz;
+                        for (int u = 0; u < GT_STAGE; u++) {
+                            g[u] = index();
+                            iz += sz_;
+                            int cy = 0;
+                            if (iz >= ez) { iz -= ez; cy = 1; }
+                            iy += sy + cy;
+                            int cx = 0;
+                            if (iy >= nys) { iy -= nys; cx = 1; }
+                            ix += sx + cx;
+                            gi += gstep + (cy ? gcy : 0u) + (cx ? gcx : 0u);
                         }
-                        iz += sz_;
-                        int cy = 0;
-                        if (iz >= ez) { iz -= ez; cy = 1; }
-                        iy += sy + cy;
-                        int cx = 0;
-                        if (iy >= nys) { iy -= nys; cx = 1; }
-                        ix += sx + cx;
-                        gi += gstep + (cy ? gcy : 0u) + (cx ? gcx : 0u);
-                    }
-                    float a[GT_STAGE];
+                        float a[GT_STAGE];
 #pragma unroll
-                    for (int u = 0; u < GT_STAGE; u++) {
-                        // unconditional loads (the slab's first cell past its end) keep all
-                        // GT_STAGE of them in flight together
-                        const unsigned gg = i0 + u * GT_NT < vol ? g[u] : g0;
-                        a[u] = psi[gg];
-                    }
+                        for (int u = 0; u < GT_STAGE; u++) {
+                            // unconditional loads (the slab's first cell past its end) keep
+                            // all GT_STAGE of them in flight together
+                            const unsigned gg = i0 + u * GT_NT < vol ? g[u] : g0;
+                            a[u] = psi[gg];
+                        }
 #pragma unroll
-                    for (int u = 0; u < GT_STAGE; u++)
-                        if (i0 + u * GT_NT < vol) sp[i0 + u * GT_NT] = a[u];
+                        for (int u = 0; u < GT_STAGE; u++)
+                            if (i0 + u * GT_NT < vol) sp[i0 + u * GT_NT] = a[u];
+                    }
+                };
+                if (inside) {
+                    // no periodic wrap: the grid index advances with the walk
+                    stage([&]() { return gi; });
+                } else {
+                    // ez <= ng + 3 and the staged origin >= -ng/2 - 1: one compare-and-add
+                    // wraps every coordinate
+                    stage([&]() {
+                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
+                        return ((unsigned)gx * ung + (unsigned)gy) * ung + (unsigned)gz;
+                    });
                 }
             }
             __syncthreads();
