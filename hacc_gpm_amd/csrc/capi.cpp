@@ -72,6 +72,7 @@ struct hgpm_ctx {
     int* flag = nullptr;
     // state
     bool have_particles = false, have_density = false, have_force = false;
+    bool dens_f32 = false; // the density may be the FP32 grid in psi's buffer (f32_deposit, fused)
     // per-kernel timing
     bool timing = false;
     std::vector<std::string> names;
@@ -237,9 +238,12 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     return b;
 }
 
-// Density of the current positions into B (int64, every cell written) and its total
-// into ctl[0]: windows -> owner blocks -> deferred tiles.
-int f32_deposit(hgpm_ctx* c) {
+// Density of the current positions and its total per box: windows -> owner blocks ->
+// deferred tiles.  The density is the int64 grid B (every cell written); with fused
+// (a step whose solve follows at once) the owner pass writes it as the FP32 grid the
+// forward z transform takes, into psi's buffer, when no tile was deferred (decided on
+// the device).
+int f32_deposit(hgpm_ctx* c, bool fused = false) {
     const DepositBuffers b = dep_buffers(c);
     Timed t(c, "deposit");
     launch_zero_u64(c->ctl, HGPM_DEP_CTL, c->st);
@@ -248,14 +252,19 @@ int f32_deposit(hgpm_ctx* c) {
         launch_deposit_windows(c->pos, c->N * c->nbox, c->ng, ref_mass(c), b, c->st);
         c->boxes_valid = true;
     }
+    if (fused) {
+        Timed t2(c, "deposit_totals");
+        launch_deposit_totals(c->N * c->nbox, b, c->st);
+    }
     {
         Timed t2(c, "deposit_owner");
-        launch_deposit_owner(c->ng, (long long*)c->B, b, c->st);
+        launch_deposit_owner(c->ng, (long long*)c->B, fused ? c->psi : nullptr, b, c->st);
     }
     {
         Timed t3(c, "deposit_deferred");
         launch_deposit_deferred(c->pos, c->N * c->nbox, c->ng, ref_mass(c), (long long*)c->B, b, c->st);
     }
+    c->dens_f32 = fused;
     LAUNCHCHK(c);
     return HGPM_OK;
 }
@@ -267,7 +276,8 @@ int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {
         Timed t(c, "fft_fwd_z_r2c");
-        f->fwd_z_r2c((const long long*)c->B, c->total, c->A, c->twf, c->nbox, c->st);
+        f->fwd_z_r2c((const long long*)c->B, c->dens_f32 ? c->psi : nullptr, c->ctl, c->total, c->A, c->twf,
+                     c->nbox, c->st);
     }
     {
         Timed t(c, "fft_fwd_y");
@@ -764,15 +774,24 @@ int hgpm_update_positions(hgpm_ctx* c, double prefactor) {
     return HGPM_OK;
 }
 
-int hgpm_cic(hgpm_ctx* c) {
-    if (!c) return HGPM_ERR_ARG;
+} // extern "C"
+namespace {
+// fused: the solve follows at once (hgpm_step), the density need not be the int64 grid
+int cic_impl(hgpm_ctx* c, bool fused) {
     int rc = need_particles(c);
     if (rc) return rc;
-    rc = c->prec == HGPM_FP32 ? f32_deposit(c) : f64_deposit(c, c->pos64);
+    rc = c->prec == HGPM_FP32 ? f32_deposit(c, fused) : f64_deposit(c, c->pos64);
     if (rc) return rc;
     c->have_density = true;
     c->have_force = false;
     return HGPM_OK;
+}
+} // namespace
+extern "C" {
+
+int hgpm_cic(hgpm_ctx* c) {
+    if (!c) return HGPM_ERR_ARG;
+    return cic_impl(c, false);
 }
 
 int hgpm_get_density(hgpm_ctx* c, int64_t* rho) {
@@ -812,6 +831,7 @@ int hgpm_set_density(hgpm_ctx* c, const int64_t* rho) {
     HIPCHK(c, hipMemcpyAsync(c->total, &ut, sizeof(ut), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->have_density = true;
+    c->dens_f32 = false;
     c->have_force = false;
     return HGPM_OK;
 }
@@ -916,7 +936,7 @@ int hgpm_step(hgpm_ctx* c, const hgpm_step_coeffs* k) {
     int rc = need_particles(c);
     if (rc) return rc;
     if (!k->skip_drift0 && (rc = hgpm_update_positions(c, k->drift0))) return rc;
-    if ((rc = hgpm_cic(c))) return rc;
+    if ((rc = cic_impl(c, true))) return rc;
     if ((rc = hgpm_solve_gradient(c))) return rc;
     return hgpm_kick_drift(c, k->kick_dt, k->kick_fscal, k->drift1);
 }

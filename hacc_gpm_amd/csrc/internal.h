@@ -22,8 +22,10 @@ struct FftOps {
     // the same shape stored back to back; totals[b] is box b's deposited mass
     int hz;
     void (*pass_half)(float2* d, int axis, int dir, const float2* tw, int variant, int nbox, hipStream_t s);
-    void (*fwd_z_r2c)(const long long* rho, const unsigned long long* totals, float2* out, const float2* tw,
-                      int nbox, hipStream_t s);
+    // rhof non-null: the density is the FP32 grid rhof when ctl[1] == 0 (no deferred
+    // tiles, see launch_deposit_owner), else the int64 grid rho
+    void (*fwd_z_r2c)(const long long* rho, const float* rhof, const unsigned long long* ctl,
+                      const unsigned long long* totals, float2* out, const float2* tw, int nbox, hipStream_t s);
     // forward x, psi^ = g rho^, inverse x, in place (the potential's half spectrum)
     void (*x_psi)(float2* h1, const float2* twf, const float2* twr, const float* hs, int variant, int nbox,
                   hipStream_t s);
@@ -83,7 +85,11 @@ struct DepositBuffers {
 };
 void launch_deposit_windows(const PosPackH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
-void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s);
+// rhof non-null: if no tile was deferred, write the FP32 density (q - mean) * 2^-24 to
+// rhof instead of the int64 density to rho (needs the totals: launch_deposit_totals first)
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s);
+// totals[box] = sum of the box's tile totals
+void launch_deposit_totals(int64_t np3, const DepositBuffers& b, hipStream_t s);
 void launch_deposit_deferred(const PosPackH* pos, int64_t np3, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s);
 // variant 0: 512-thread workgroups on half tiles; 1: 256-thread workgroups on quarter tiles

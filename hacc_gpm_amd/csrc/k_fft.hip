@@ -92,18 +92,23 @@ __global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __re
 // for RC real rows: rows 2c and 2c+1 form the complex column c = a + i b, transformed
 // once; A[k] = (C[k] + conj C[N-k]) / 2 and B[k] = (C[k] - conj C[N-k]) / 2i give the
 // half spectra, k = 0..N/2, written with pitch HZ (padding written as zero).
+// rhof: see fwd_z_r2c (the owner pass wrote the same FP32 values when no tile was deferred).
 template <int RC>
 __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __restrict rho,
+                                                         const float* __restrict rhof,
+                                                         const unsigned long long* __restrict ctl,
                                                          const unsigned long long* __restrict total,
                                                          float2* __restrict out, const float2* __restrict tw) {
     constexpr int PC = RC / 2;
     __shared__ float2 lds[lds_elems<N, PC>()];
     float2* spec = lds; // the last stage has read all its inputs before the first output
     const size_t row0 = (size_t)blockIdx.x * RC;
+    const bool f32 = rhof != nullptr && ctl[1] == 0ull;
     // total of this row's box: a batch holds several boxes of the same shape
     const double mean = (double)(long long)total[row0 / ((size_t)N * N)] / ((double)N * N * N);
     fft_tile<float2, N, PC, NT, -1, false, true>(
         [&](int, int, int n, int c) {
+            if (f32) return make_float2(rhof[(row0 + 2 * c) * N + n], rhof[(row0 + 2 * c + 1) * N + n]);
             const double qa = (double)rho[(row0 + 2 * c) * N + n];
             const double qb = (double)rho[(row0 + 2 * c + 1) * N + n];
             return make_float2((float)((qa - mean) * (1.0 / 16777216.0)), (float)((qb - mean) * (1.0 / 16777216.0)));
@@ -254,11 +259,11 @@ void fwd_z_density(const long long* rho, const unsigned long long* total, float2
     k_rows_density<rc32()><<<nb, NT, 0, s>>>(rho, total, out, tw);
 }
 // nbox grids of the same shape back to back (rows of every box in one launch)
-void fwd_z_r2c(const long long* rho, const unsigned long long* totals, float2* out, const float2* tw, int nbox,
-               hipStream_t s) {
+void fwd_z_r2c(const long long* rho, const float* rhof, const unsigned long long* ctl,
+               const unsigned long long* totals, float2* out, const float2* tw, int nbox, hipStream_t s) {
     constexpr int RC = 2 * rc32();
     unsigned nb = (unsigned)((size_t)nbox * N * N / RC);
-    k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, totals, out, tw);
+    k_rows_r2c_density<RC><<<nb, NT, 0, s>>>(rho, rhof, ctl, totals, out, tw);
 }
 void inv_z_c2r_real(const float2* hz, float* out, const float2* tw, int nbox, hipStream_t s) {
     constexpr int RC = 2 * rc32();

@@ -517,10 +517,15 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
 // 64 j, decoded once per chunk) and walks the chunk's x planes, so a cell costs an
 // add for its arena address and one for its LDS address; OWN_U planes are loaded
 // before the first LDS add so that their loads are in flight together.
+// rhof (may be null): when no tile was deferred (ctl[1] == 0) the block is written as
+// FP32 (q - mean) * 2^-24 to rhof, the form the forward z transform takes, instead of
+// int64 q to rho; mean = totals[box] / ng^3 (the totals of every tile are final).
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __restrict arena,
                                                           const uint4* __restrict slots,
                                                           unsigned* __restrict count, int nslots, int ng,
-                                                          unsigned long long* __restrict rho, int nblk,
+                                                          unsigned long long* __restrict rho, float* __restrict rhof,
+                                                          const unsigned long long* __restrict ctl,
+                                                          const unsigned long long* __restrict totals, int nblk,
                                                           int64_t gcells) {
     constexpr int OWN_J = OB * OB / 64; // plane slots per lane (a plane holds <= OB^2 cells)
     __shared__ unsigned long long acc[OB * OB * OB];
@@ -530,6 +535,9 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
     const int blk = (int)xcd_remap(blockIdx.x, gridDim.x); // over every box of the batch
     const int box = blk / nblk, lb = blk - box * nblk;
     rho += (int64_t)box * gcells;
+    const bool f32 = rhof != nullptr && ctl[1] == 0ull;
+    const double mean = f32 ? (double)(long long)totals[box] / (double)gcells : 0.0;
+    if (f32) rhof += (int64_t)box * gcells;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // every slot descriptor is loaded together with the count (the unused ones are
     // ignored), so the two loads cost one memory latency
@@ -589,10 +597,18 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
     // thread tid writes (x, y, z) = (k, tid / OB, tid % OB) of the block for k = 0..OB-1
     static_assert(OWN_NT == OB * OB, "one (y, z) column of the block per thread");
     const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
-    unsigned long long* out = rho + gidx(ng, bx * OB, by * OB + (tid >> 4), bz * OB + (tid & 15));
+    const int64_t at = gidx(ng, bx * OB, by * OB + (tid >> 4), bz * OB + (tid & 15));
     const int64_t plane = (int64_t)ng * ng;
+    if (f32) {
 #pragma unroll 4
-    for (int k = 0; k < OB; k++) out[k * plane] = acc[k * OB * OB + tid];
+        for (int k = 0; k < OB; k++) {
+            const double q = (double)(long long)acc[k * OB * OB + tid];
+            rhof[at + k * plane] = (float)((q - mean) * (1.0 / 16777216.0));
+        }
+    } else {
+#pragma unroll 4
+        for (int k = 0; k < OB; k++) rho[at + k * plane] = acc[k * OB * OB + tid];
+    }
 }
 
 // Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
@@ -994,12 +1010,17 @@ void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, 
                                                        (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
                                                        b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
 }
-void launch_deposit_owner(int ng, long long* rho, const DepositBuffers& b, hipStream_t s) {
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
     const int nblk = (int)(nb * nb * nb);
     k_deposit_owner<<<nb * nb * nb * b.nbox, OWN_NT, 0, s>>>(b.arena, (const uint4*)b.slots, b.count, b.nslots, ng,
-                                                             (unsigned long long*)rho, nblk,
+                                                             (unsigned long long*)rho, rhof, b.ctl, b.totals, nblk,
                                                              (int64_t)ng * ng * ng);
+}
+void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
+    if (!n) return;
+    const int ntb = (int)(n / DEP_TILE / b.nbox);
+    k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
 }
 void launch_deposit_deferred(const PosPackH* pos, int64_t n, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s) {

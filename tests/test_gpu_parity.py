@@ -358,7 +358,10 @@ def test_initial_conditions(hg, orc, tmp_path, prec):
 @pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64)])
 def test_step_fused_deposit_matches_separate_calls(hg, orc, ng, np_):
     """hgpm_step (merged drifts, an output step with half drifts and P(k) in between)
-    must leave particles and density bit-identical to the step done by separate calls."""
+    must leave particles and density bit-identical to the step done by separate calls.
+    The step's owner pass hands the solve an FP32 density when no tile is deferred; a
+    second pair of contexts with a tiny deposit arena defers tiles and takes the int64
+    route inside the step."""
     case(f"fused step ng={ng} np={np_}")
     rng = np.random.default_rng(ng + np_)
     n_steps = 6
@@ -368,26 +371,32 @@ def test_step_fused_deposit_matches_separate_calls(hg, orc, ng, np_):
     sched = hg.schedule(p, n_steps, merge=True, output_after=out_after)
     cell, off = lattice_split(rng, np_, ng, 1.5, outliers=2)
     vel = rng.normal(0, 2.0, (np_ ** 3, 3)).astype(np.float32)
-    with hg.MemoryManager(p) as m1, hg.MemoryManager(p) as m2:
-        m1.set_particles_split(cell, off, vel)
-        m2.set_particles_split(cell, off, vel)
-        for s, co in enumerate(sched):
-            m1.step(co)
-            if not co.skip_drift0:
-                m2.update_positions(co.drift0)
+    for arena in (None, 1):
+        with hg.MemoryManager(p) as m1, hg.MemoryManager(p) as m2:
+            if arena is not None:
+                m1.set_tunable("deposit_arena", arena)
+                m2.set_tunable("deposit_arena", arena)
+            m1.set_particles_split(cell, off, vel)
+            m2.set_particles_split(cell, off, vel)
+            for s, co in enumerate(sched):
+                m1.step(co)
+                if not co.skip_drift0:
+                    m2.update_positions(co.drift0)
+                m2.cic()
+                m2.solve_gradient()
+                m2.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
+                if out_after[s]:
+                    k1, pk1, c1 = m1.power_spectrum(0)
+                    k2, pk2, c2 = m2.power_spectrum(0)
+                    assert np.array_equal(pk1, pk2) and np.array_equal(c1, c2)
+                a, b = m1.get_particles_split(), m2.get_particles_split()
+                for x, y in zip(a, b):
+                    assert np.array_equal(x, y), s
+            if arena is not None:
+                assert m1.get_tunable("deposit_deferred") > 0
+            m1.cic()
             m2.cic()
-            m2.solve_gradient()
-            m2.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
-            if out_after[s]:
-                k1, pk1, c1 = m1.power_spectrum(0)
-                k2, pk2, c2 = m2.power_spectrum(0)
-                assert np.array_equal(pk1, pk2) and np.array_equal(c1, c2)
-            a, b = m1.get_particles_split(), m2.get_particles_split()
-            for x, y in zip(a, b):
-                assert np.array_equal(x, y), s
-        m1.cic()
-        m2.cic()
-        assert np.array_equal(m1.get_density(), m2.get_density())
+            assert np.array_equal(m1.get_density(), m2.get_density())
 
 
 # -------------------------------------------------------------- full run ----
