@@ -45,7 +45,7 @@ struct hgpm_ctx {
     float* vel = nullptr;
     float2* A = nullptr;             // spectrum (8 B per cell) / half spectra of the solve
     float2* B = nullptr;             // int64 density (8 B per cell)
-    float* psi = nullptr;            // potential of the solve (4 B per cell); F = its central differences
+    float* psi = nullptr;            // half the potential of the solve (4 B per cell); F = its differences
     unsigned long long* ctl = nullptr;    // deposit control words (shared by the boxes)
     unsigned long long* totals = nullptr; // deposited mass per box
     unsigned long long* total = nullptr;  // = totals (of the viewed box, see BoxView)
@@ -269,9 +269,10 @@ int f32_deposit(hgpm_ctx* c, bool fused = false) {
     return HGPM_OK;
 }
 
-// Poisson solve on half spectra: density (B, int64) -> rho^ (A, pitch hz) -> psi^ =
-// g rho^ -> potential psi (real, 4 B per cell).  The forces are the central differences
-// of psi, formed by the gather (and by hgpm_get_force).
+// Poisson solve on half spectra: density (B, int64, or FP32 in psi's buffer) -> rho^ (A,
+// pitch hz) -> (g/2) rho^ -> half the potential, psi (real, 4 B per cell).  The forces
+// are the central differences of the potential, psi[c + e_d] - psi[c - e_d], formed by
+// the gather (and by hgpm_get_force).
 int f32_solve(hgpm_ctx* c) {
     const FftOps* f = c->fft;
     {

@@ -13,13 +13,14 @@
 // and every pass after the first touches half the data:
 //   r2c   : pairs of real rows a, b as one complex transform of a + i b, split into
 //           the half spectra of a and b;
-//   x_psi : forward x transform, psi^ = g rho^ (the Green's function), inverse x
-//           transform, in place;
-//   c2r   : psi rows in pairs as one complex transform of the Hermitian extension of
-//           psi^_a + i psi^_b.
+//   x_psi : forward x transform, phi^ = (g/2) rho^ (half the Green's function), inverse
+//           x transform, in place;
+//   c2r   : phi rows in pairs as one complex transform of the Hermitian extension of
+//           phi^_a + i phi^_b.
 // The reference's gradient, F_d^ = -i sin(k_d) G rho^ with G = -g, is F_d^ = i sin(k_d)
-// psi^: the central difference F_d[n] = (psi[n + e_d] - psi[n - e_d]) / 2 of the real
-// potential, which the force gather forms from psi (one inverse transform, not three).
+// psi^ with psi = IFFT(g rho^): the central difference F_d[n] = (psi[n + e_d] -
+// psi[n - e_d]) / 2 = phi[n + e_d] - phi[n - e_d] of the real field phi = psi / 2, which
+// the force gather forms (one inverse transform, not three).
 #include "fft.h"
 #include "internal.h"
 
@@ -130,12 +131,12 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __rest
 }
 
 // Half-spectrum x pass of the potential.  In: h1 = rho^ after the z and y forward
-// passes.  Forward x transform, psi^ = g rho^ with g = (0.25/N^3)/sum_d sin^2(k_d/2),
-// g(0) = 0, inverse x transform, written back to h1.  The reference's Green's function
+// passes.  Forward x transform, phi^ = (g/2) rho^ with g = (0.25/N^3)/sum_d sin^2(k_d/2),
+// g(0) = 0 (gcoef = 0.125/N^3), inverse x transform, written back to h1.  The reference's Green's function
 // G = (0.5/N^3)/(sum_d cos k_d - 3) = -g and gradient -i sin(k_d) (kspace_solve_gradient,
 // src/pm_solver/cuda/kspace_solver.cu:55-90; calcGreens, src/pm_solver/cuda/calcGreens.hpp:11-26;
-// get_gradient, src/pm_solver/kernels.hpp) give F_d^ = i sin(k_d) psi^, formed in real space
-// by the gather.  hs[m] = sin^2(pi m/N).
+// get_gradient, src/pm_solver/kernels.hpp) give F_d^ = i sin(k_d) psi^ = 2 i sin(k_d) phi^,
+// formed in real space by the gather.  hs[m] = sin^2(pi m/N).
 template <int CC, int NTC>
 __global__ __launch_bounds__(NTC) void k_x_psi(float2* __restrict h1, const float2* __restrict twf,
                                                const float2* __restrict twr, const float* __restrict hs,
@@ -273,7 +274,7 @@ void inv_z_c2r_real(const float2* hz, float* out, const float2* tw, int nbox, hi
 template <int CC, int NTC>
 void x_psi_launch(float2* h1, const float2* twf, const float2* twr, const float* hs, int nbox, hipStream_t s) {
     unsigned nb = (unsigned)((size_t)nbox * N * (HZ / CC));
-    const float gcoef = (float)(0.25 / ((double)N * N * N));
+    const float gcoef = (float)(0.125 / ((double)N * N * N)); // g / 2: the stored field is psi / 2
     k_x_psi<CC, NTC><<<nb, NTC, 0, s>>>(h1, twf, twr, hs, gcoef);
 }
 void x_psi(float2* h1, const float2* twf, const float2* twr, const float* hs, int variant, int nbox, hipStream_t s) {

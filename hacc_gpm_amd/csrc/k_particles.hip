@@ -73,8 +73,9 @@ template <bool SMALL>
 __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, float mass) {
     float w = dx * dy;
     w = w * dz;
-    w = w * mass;
-    const float q = w * 16777216.0f;
+    // (w * mass) * 2^24 in one product: scaling by 2^24 is exact, so w * (mass * 2^24)
+    // rounds to the same value, and where w * mass would be subnormal both round to 0
+    const float q = w * (mass * 16777216.0f);
     if constexpr (SMALL) return (long long)(int)rintf(q);
     else return (long long)__float2ll_rn(q);
 }
@@ -637,33 +638,34 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosPack* __re
     }
 }
 
-// totals[b] = sum of the totals of box b's tiles (one workgroup per box; integer sum,
-// order-free).
-__global__ __launch_bounds__(1024) void k_deposit_total(const unsigned long long* __restrict tile_total,
-                                                        int64_t ntb, unsigned long long* __restrict totals) {
-    __shared__ unsigned long long s[1024 / 64];
+// totals[b] = sum of the totals of box b's tiles: DEP_TOT_WG workgroups per box each
+// add their share (integer sums, order-free) to a zeroed totals[b].
+constexpr int DEP_TOT_WG = 32;
+__global__ __launch_bounds__(256) void k_deposit_total(const unsigned long long* __restrict tile_total,
+                                                       int64_t ntb, unsigned long long* __restrict totals) {
+    __shared__ unsigned long long s[256 / 64];
+    const int box = blockIdx.x / DEP_TOT_WG, part = blockIdx.x % DEP_TOT_WG;
     unsigned long long t = 0;
-    tile_total += (int64_t)blockIdx.x * ntb;
-    for (int64_t i = threadIdx.x; i < ntb; i += 1024) t += tile_total[i];
+    tile_total += (int64_t)box * ntb;
+    for (int64_t i = (int64_t)part * 256 + threadIdx.x; i < ntb; i += (int64_t)DEP_TOT_WG * 256) t += tile_total[i];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = t;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0;
-        for (int w = 0; w < 1024 / 64; w++) a += s[w];
-        totals[blockIdx.x] = a;
-    }
+    if (threadIdx.x == 0) atomicAdd(&totals[box], s[0] + s[1] + s[2] + s[3]);
 }
 
-// Force of a cell from the potential psi: the central difference
-// F_d = (psi[c + e_d] - psi[c - e_d]) / 2, the real-space form of the reference's
-// spectral gradient -i sin(k_d) (kspace_solve_gradient, src/pm_solver/cuda/kspace_solver.cu:55-90).
-// One expression for the gather and the force grid, so both give the same bits.
-__device__ __forceinline__ float central_diff(float up, float down) { return 0.5f * (up - down); }
+// Force of a cell from the stored field phi = psi / 2 (half the potential, so that no
+// factor 1/2 is applied per force): the central difference
+// F_d = (psi[c + e_d] - psi[c - e_d]) / 2 = phi[c + e_d] - phi[c - e_d], the real-space
+// form of the reference's spectral gradient -i sin(k_d) (kspace_solve_gradient,
+// src/pm_solver/cuda/kspace_solver.cu:55-90).  One expression for the gather and the
+// force grid, so both give the same bits.
+__device__ __forceinline__ float central_diff(float up, float down) { return up - down; }
 
 // Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
 // summed over the 8 corners in (x,y,z) order, then optionally the drift.  The force of
-// each corner is the central difference of the potential psi (central_diff).  A
+// each corner is the central difference of the potential (central_diff of psi, the
+// stored half potential).  A
 // workgroup takes GT_TILE consecutive particles (part of a Lagrangian tile, a compact
 // block of the lattice).  The potential under the bounding box of the particles' base
 // cells, widened by one cell below and two above (the corners and their neighbours),
@@ -956,7 +958,7 @@ __global__ void k_fold_split(PosPack* __restrict pos, int64_t n, int ng) {
     pos[i] = pack(p);
 }
 
-// Force of every cell (x, y, z order, 3 floats per cell) from the potential of one box.
+// Force of every cell (x, y, z order, 3 floats per cell) from the stored half potential of one box.
 __global__ void k_force_from_psi(const float* __restrict psi, float* __restrict out, int ng) {
     const int64_t g3 = (int64_t)ng * ng * ng;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1020,7 +1022,8 @@ void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuff
 void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
     const int ntb = (int)(n / DEP_TILE / b.nbox);
-    k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
+    k_zero_u64<<<1, 64, 0, s>>>(b.totals, (size_t)b.nbox);
+    k_deposit_total<<<b.nbox * DEP_TOT_WG, 256, 0, s>>>(b.tile_total, ntb, b.totals);
 }
 void launch_deposit_deferred(const PosPackH* pos, int64_t n, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s) {
@@ -1036,7 +1039,7 @@ void launch_deposit_deferred(const PosPackH* pos, int64_t n, int ng, float mass,
         k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass,
                                                           (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total,
                                                           ntb, gc);
-    k_deposit_total<<<b.nbox, 1024, 0, s>>>(b.tile_total, ntb, b.totals);
+    launch_deposit_totals(n, b, s);
 }
 template <int NT>
 void gather_launch(PosPackH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT, double fscal,
