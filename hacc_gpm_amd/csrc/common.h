@@ -35,6 +35,23 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
     return x * per + (x < rem ? x : rem) + i;
 }
 
+// Wave-wide minimum / maximum (64 lanes, every lane active), returned to every lane:
+// DPP steps combine lanes within a row of 16 (quad permutations, half-row and row
+// mirrors), then row broadcasts carry rows 0 -> 1, 2 -> 3 and row 1 -> rows 2, 3; lane 63
+// holds the result.  Lanes not written by a masked DPP step keep their own value.
+template <bool MIN> __device__ __forceinline__ int wave_minmax(int v) {
+    auto op = [](int a, int b) { return MIN ? min(a, b) : max(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false)); // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false)); // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false)); // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false)); // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min(int v) { return wave_minmax<true>(v); }
+__device__ __forceinline__ int wave_max(int v) { return wave_minmax<false>(v); }
+
 // Grid index of cell (x, y, z): serial::get_index order (src/haccgpm.hpp:423-429).
 __device__ __forceinline__ int64_t gidx(int ng, int x, int y, int z) {
     return ((int64_t)x * ng + y) * ng + z;

@@ -166,20 +166,21 @@ __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int6
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            for (int off = 32; off > 0; off >>= 1) {
-                mn[h][d] = min(mn[h][d], __shfl_xor(mn[h][d], off, 64));
-                mx[h][d] = max(mx[h][d], __shfl_xor(mx[h][d], off, 64));
-            }
-            if (lane == 0) { red[6 * h + d][wv] = mn[h][d]; red[6 * h + 3 + d][wv] = mx[h][d]; }
+            const int a = wave_min(mn[h][d]), b = wave_max(mx[h][d]);
+            if (lane == 0) { red[6 * h + d][wv] = a; red[6 * h + 3 + d][wv] = b; }
         }
+    __syncthreads();
+    // threads 0..11 reduce one quantity each over the waves; the others read the results
+    if (tid < 12) {
+        const bool is_min = (tid % 6) < 3;
+        int v = red[tid][0];
+        for (int w = 1; w < DEP_NT / 64; w++) v = is_min ? min(v, red[tid][w]) : max(v, red[tid][w]);
+        red[tid][0] = v;
+    }
     __syncthreads();
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        int a0 = red[d][0], b0 = red[3 + d][0], a1 = red[6 + d][0], b1 = red[9 + d][0];
-        for (int w = 1; w < DEP_NT / 64; w++) {
-            a0 = min(a0, red[d][w]); b0 = max(b0, red[3 + d][w]);
-            a1 = min(a1, red[6 + d][w]); b1 = max(b1, red[9 + d][w]);
-        }
+        const int a0 = red[d][0], b0 = red[3 + d][0], a1 = red[6 + d][0], b1 = red[9 + d][0];
         lo[d] = min(a0, a1);
         e[d] = max(b0, b1) - lo[d] + 2;
         if (hb) {
@@ -682,9 +683,10 @@ constexpr int GT_STAGE = 8;                // staged cells per thread per load r
 constexpr int GT_GMAX = 16 * (1024 + 3);   // 64 KB of FP32 potential
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
-// (at least 4 waves per SIMD: two 512-thread workgroups per CU, as the LDS budget allows)
+// (registers for the two workgroups per CU the LDS budget allows: 4 waves per SIMD at
+// 512 threads, 2 at 256)
 template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
-__global__ __launch_bounds__(GT_NT, 4) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
+__global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float* __restrict psi, double deltaT,
                                                        double fscal, int do_drift, float pf,
                                                        const int4* __restrict boxes, int64_t npb, int64_t gcells) {
