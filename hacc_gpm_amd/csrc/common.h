@@ -51,6 +51,22 @@ template <bool MIN> __device__ __forceinline__ int wave_minmax(int v) {
 }
 __device__ __forceinline__ int wave_min(int v) { return wave_minmax<true>(v); }
 __device__ __forceinline__ int wave_max(int v) { return wave_minmax<false>(v); }
+// The same for two signed 16-bit values packed in one word (low, high), each reduced
+// with max: one DPP sequence for two quantities.
+typedef short hgpm_s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int pk_max16(int a, int b) {
+    return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(hgpm_s16x2, a),
+                                                             __builtin_bit_cast(hgpm_s16x2, b)));
+}
+__device__ __forceinline__ int wave_max_pk16(int v) {
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
 
 // Grid index of cell (x, y, z): serial::get_index order (src/haccgpm.hpp:423-429).
 __device__ __forceinline__ int64_t gidx(int ng, int x, int y, int z) {

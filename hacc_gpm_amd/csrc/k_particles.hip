@@ -162,12 +162,17 @@ __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int6
         mn[h][1] = min(mn[h][1], R.ry[k]); mx[h][1] = max(mx[h][1], R.ry[k]);
         mn[h][2] = min(mn[h][2], R.rz[k]); mx[h][2] = max(mx[h][2], R.rz[k]);
     }
+    // relative cells lie in [-ng/2, ng/2) with ng <= 1024: -min and max fit 16 bits each
+    // and are reduced in pairs
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const int a = wave_min(mn[h][d]), b = wave_max(mx[h][d]);
-            if (lane == 0) { red[6 * h + d][wv] = a; red[6 * h + 3 + d][wv] = b; }
+            const int pk = wave_max_pk16((int)(((unsigned)(-mn[h][d]) & 0xFFFFu) | ((unsigned)mx[h][d] << 16)));
+            if (lane == 0) {
+                red[6 * h + d][wv] = -(int)(short)(pk & 0xFFFF);
+                red[6 * h + 3 + d][wv] = pk >> 16;
+            }
         }
     __syncthreads();
     // threads 0..11 reduce one quantity each over the waves; the others read the results
@@ -596,20 +601,23 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
         }
     }
     __syncthreads();
-    // thread tid writes (x, y, z) = (k, tid / OB, tid % OB) of the block for k = 0..OB-1
-    static_assert(OWN_NT == OB * OB, "one (y, z) column of the block per thread");
+    // thread tid writes (x, y, z) = (k, t / OB, t % OB), t = tid % OB^2, of the block for
+    // k = tid / OB^2, tid / OB^2 + KS, ...
+    constexpr int KS = OWN_NT / (OB * OB);
+    static_assert(OWN_NT % (OB * OB) == 0 && OB % KS == 0, "whole (y, z) planes per pass");
+    const int t = tid % (OB * OB), k0 = tid / (OB * OB);
     const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
-    const int64_t at = gidx(ng, bx * OB, by * OB + (tid >> 4), bz * OB + (tid & 15));
+    const int64_t at = gidx(ng, bx * OB, by * OB + (t >> 4), bz * OB + (t & 15));
     const int64_t plane = (int64_t)ng * ng;
     if (f32) {
 #pragma unroll 4
-        for (int k = 0; k < OB; k++) {
-            const double q = (double)(long long)acc[k * OB * OB + tid];
+        for (int k = k0; k < OB; k += KS) {
+            const double q = (double)(long long)acc[k * OB * OB + t];
             rhof[at + k * plane] = (float)((q - mean) * (1.0 / 16777216.0));
         }
     } else {
 #pragma unroll 4
-        for (int k = 0; k < OB; k++) rho[at + k * plane] = acc[k * OB * OB + tid];
+        for (int k = k0; k < OB; k += KS) rho[at + k * plane] = acc[k * OB * OB + t];
     }
 }
 
