@@ -685,10 +685,10 @@ __device__ __forceinline__ float central_diff(float up, float down) { return up 
 // Workgroup shapes (threads x particles per thread):
 //   variant 0: 512 x 4 = half a Lagrangian tile (8 x 16 x 16)
 //   variant 1: 256 x 4 = a quarter tile (4 x 16 x 16)
-// The budget must hold one base plane and row with their halo, 4 x 4 x (ng + 3) cells,
-// for ng <= 1024.
-constexpr int GT_STAGE = 8;                // staged cells per thread per load round
-constexpr int GT_GMAX = 16 * (1024 + 3);   // 64 KB of FP32 potential
+// The budget must hold one base plane and row with their halo, 4 x 4 rows of ng + 3
+// cells padded to a multiple of 4, for ng <= 1024.
+constexpr int GT_SEGS = 2;                 // staged 4-cell segments per thread per load round
+constexpr int GT_GMAX = 16 * (1024 + 4);   // 64 KB of FP32 potential
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
 // (registers for the two workgroups per CU the LDS budget allows: 4 waves per SIMD at
@@ -700,8 +700,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                                                        const int4* __restrict boxes, int64_t npb, int64_t gcells) {
     constexpr int GT_TILE = GT_NT * GT_PPT;
     static_assert(DEP_TILE % GT_TILE == 0 && GT_TILE <= DEP_TILE / 2, "tile");
-    static_assert(GMAX >= 16 * (1024 + GT_HALO), "one base plane and row of the widest window");
-    __shared__ float sp[GMAX];
+    static_assert(GMAX >= 16 * (1024 + 4), "one base plane and row of the widest window, padded rows");
+    __shared__ __align__(16) float sp[GMAX];
     __shared__ int red[6][GT_NT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
@@ -759,77 +759,69 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
             b[d] = c - a + 1; // base cells per axis
         }
     }
-    const int ez = b[2] + GT_HALO; // <= ng + 3
-    // slab shape: ty base rows (ty + 3 staged) with 4 (ty + 3) ez <= GMAX, then tx base
-    // planes (tx + 3 staged) with (tx + 3) (ty + 3) ez <= GMAX; ez <= ng + 3 <= 1027
+    const int ez = b[2] + GT_HALO;   // staged cells per row, <= ng + 3
+    const int ezp = (ez + 3) & ~3;   // LDS row pitch: whole 4-cell segments, <= ng + 4
+    const int nseg = ezp >> 2;
+    // slab shape: ty base rows (ty + 3 staged) with 4 (ty + 3) ezp <= GMAX, then tx base
+    // planes (tx + 3 staged) with (tx + 3) (ty + 3) ezp <= GMAX; ezp <= ng + 4 <= 1028
     // keeps both >= 1
-    int ty = min(b[1], GMAX / (4 * ez) - GT_HALO);
+    int ty = min(b[1], GMAX / (4 * ezp) - GT_HALO);
     ty = (b[1] + (b[1] + ty - 1) / ty - 1) / ((b[1] + ty - 1) / ty); // even split
-    int tx = min(b[0], GMAX / ((ty + GT_HALO) * ez) - GT_HALO);
+    int tx = min(b[0], GMAX / ((ty + GT_HALO) * ezp) - GT_HALO);
     tx = (b[0] + (b[0] + tx - 1) / tx - 1) / ((b[0] + tx - 1) / tx);
     for (int sx0 = 0; sx0 < b[0]; sx0 += tx) {
         const int nbx = min(tx, b[0] - sx0), nxs = nbx + GT_HALO;
         for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
             const int nby = min(ty, b[1] - sy0), nys = nby + GT_HALO;
-            const int vol = nxs * nys * ez;
             {
-                const int sz_ = GT_NT % ez, syx = GT_NT / ez, sy = syx % nys, sx = syx / nys;
-                int iz = tid % ez, iy = (tid / ez) % nys, ix = (tid / ez) / nys;
-                // staged origin: one cell below the slab's first base cell on every axis
+                // Staging in 4-cell segments of the slab's rows (row = staged (x, y), ezp
+                // cells of z): item it -> row it / nseg, segment it % nseg.  Cells past ez
+                // in a row's last segment are padding, never read back.  The staged origin
+                // is one cell below the slab's first base cell on every axis.
                 const int bx0 = r0.cx + lo[0] + sx0 - 1, by0 = r0.cy + lo[1] + sy0 - 1, bz0 = r0.cz + lo[2] - 1;
-                // a slab that does not cross the periodic boundary (uniform per workgroup)
-                // advances the grid index with the walk instead of wrapping each cell
-                const bool inside = bx0 >= 0 && bx0 + nxs <= ng && by0 >= 0 && by0 + nys <= ng && bz0 >= 0 &&
-                                    bz0 + ez <= ng;
-                const unsigned ung = (unsigned)ng, ung2 = ung * ung;
-                const int wx0 = wrap_cell(bx0, ng), wy0 = wrap_cell(by0, ng), wz0 = wrap_cell(bz0, ng);
-                const unsigned g0 = ((unsigned)wx0 * ung + (unsigned)wy0) * ung + (unsigned)wz0;
-                unsigned gi = ((unsigned)(bx0 + ix) * ung + (unsigned)(by0 + iy)) * ung + (unsigned)(bz0 + iz);
-                const unsigned gstep = (unsigned)sz_ + (unsigned)sy * ung + (unsigned)sx * ung2;
-                const unsigned gcy = ung - (unsigned)ez, gcx = ung2 - (unsigned)nys * ung;
-                // the slab's cells, GT_STAGE per thread per round: all their loads are
-                // issued before the first LDS store, so a round costs one memory latency
-                auto stage = [&](auto index) {
-                    for (int i0 = tid; i0 < vol; i0 += GT_STAGE * GT_NT) {
-                        unsigned g[GT_STAGE];
+                const int wz0 = wrap_cell(bz0, ng);
+                // the padded row inside one grid row (uniform per workgroup): no z wrap
+                const bool zin = wz0 + ezp <= ng;
+                const int nitems = nxs * nys * nseg;
+                // it -> (row, segment) and row -> (x, y) by multiplication with rounded
+                // reciprocals, each quotient corrected by one step (exact for any sizes here)
+                const float rseg = 1.0f / (float)nseg, rnys = 1.0f / (float)nys;
+                for (int i0 = tid; i0 < nitems; i0 += GT_SEGS * GT_NT) {
+                    float a[GT_SEGS][4];
+                    int at[GT_SEGS];
 #pragma unroll
-                        for (int u = 0; u < GT_STAGE; u++) {
-                            g[u] = index();
-                            iz += sz_;
-                            int cy = 0;
-                            if (iz >= ez) { iz -= ez; cy = 1; }
-                            iy += sy + cy;
-                            int cx = 0;
-                            if (iy >= nys) { iy -= nys; cx = 1; }
-                            ix += sx + cx;
-                            gi += gstep + (cy ? gcy : 0u) + (cx ? gcx : 0u);
-                        }
-                        float a[GT_STAGE];
-#pragma unroll
-                        for (int u = 0; u < GT_STAGE; u++) {
-                            // unconditional loads (the slab's first cell past its end) keep
-                            // all GT_STAGE of them in flight together
-                            const unsigned gg = i0 + u * GT_NT < vol ? g[u] : g0;
-                            a[u] = psi[gg];
-                        }
-#pragma unroll
-                        for (int u = 0; u < GT_STAGE; u++)
-                            if (i0 + u * GT_NT < vol) sp[i0 + u * GT_NT] = a[u];
-                    }
-                };
-                if (inside) {
-                    // no periodic wrap: the grid index advances with the walk
-                    stage([&]() { return gi; });
-                } else {
-                    // ez <= ng + 3 and the staged origin >= -ng/2 - 1: one compare-and-add
-                    // wraps every coordinate
-                    stage([&]() {
-                        int gx = bx0 + ix, gy = by0 + iy, gz = bz0 + iz;
+                    for (int u = 0; u < GT_SEGS; u++) {
+                        // items past the end load the slab's first segment (in bounds)
+                        const int it = min(i0 + u * GT_NT, nitems - 1);
+                        int row = (int)(((float)it + 0.5f) * rseg), sg = it - row * nseg;
+                        if (sg < 0) { row -= 1; sg += nseg; } else if (sg >= nseg) { row += 1; sg -= nseg; }
+                        int x = (int)(((float)row + 0.5f) * rnys), y = row - x * nys;
+                        if (y < 0) { x -= 1; y += nys; } else if (y >= nys) { x += 1; y -= nys; }
+                        at[u] = row * ezp + 4 * sg;
+                        int gx = bx0 + x, gy = by0 + y;
                         gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
                         gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                        gz = gz < 0 ? gz + ng : (gz >= ng ? gz - ng : gz);
-                        return ((unsigned)gx * ung + (unsigned)gy) * ung + (unsigned)gz;
-                    });
+                        const float* __restrict rp = psi + ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng;
+                        if (zin) {
+                            const int z = wz0 + 4 * sg;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) a[u][c] = rp[z + c];
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < 4; c++) {
+                                // ezp <= ng + 4 and wz0 < ng: one subtraction wraps (twice for
+                                // the padding of a row that spans the whole grid)
+                                int z = wz0 + 4 * sg + c;
+                                z = z >= ng ? z - ng : z;
+                                z = z >= ng ? z - ng : z;
+                                a[u][c] = rp[z];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < GT_SEGS; u++)
+                        if (i0 + u * GT_NT < nitems)
+                            *(float4*)&sp[at[u]] = make_float4(a[u][0], a[u][1], a[u][2], a[u][3]);
                 }
             }
             __syncthreads();
@@ -851,8 +843,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 const float gy[2] = {1.0f - p.fy, p.fy};
                 const float gz[2] = {1.0f - p.fz, p.fz};
                 // staged index of the base corner; strides of x and y
-                const int sxs = nys * ez;
-                const int b0 = ((lx + 1) * nys + (ly + 1)) * ez + (lz + 1);
+                const int sxs = nys * ezp;
+                const int b0 = ((lx + 1) * nys + (ly + 1)) * ezp + (lz + 1);
                 // the 2x2x2 corners and the 24 neighbours their differences need
                 float q[2][2][2], xm[2][2], xp[2][2], ym[2][2], yp[2][2], zm[2][2], zp[2][2];
 #pragma unroll
@@ -860,13 +852,13 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
 #pragma unroll
                     for (int v = 0; v < 2; v++) {
 #pragma unroll
-                        for (int w = 0; w < 2; w++) q[u][v][w] = sp[b0 + u * sxs + v * ez + w];
-                        xm[u][v] = sp[b0 - sxs + u * ez + v];     // (x-1, y+u, z+v)
-                        xp[u][v] = sp[b0 + 2 * sxs + u * ez + v]; // (x+2, y+u, z+v)
-                        ym[u][v] = sp[b0 + u * sxs - ez + v];     // (x+u, y-1, z+v)
-                        yp[u][v] = sp[b0 + u * sxs + 2 * ez + v]; // (x+u, y+2, z+v)
-                        zm[u][v] = sp[b0 + u * sxs + v * ez - 1]; // (x+u, y+v, z-1)
-                        zp[u][v] = sp[b0 + u * sxs + v * ez + 2]; // (x+u, y+v, z+2)
+                        for (int w = 0; w < 2; w++) q[u][v][w] = sp[b0 + u * sxs + v * ezp + w];
+                        xm[u][v] = sp[b0 - sxs + u * ezp + v];     // (x-1, y+u, z+v)
+                        xp[u][v] = sp[b0 + 2 * sxs + u * ezp + v]; // (x+2, y+u, z+v)
+                        ym[u][v] = sp[b0 + u * sxs - ezp + v];     // (x+u, y-1, z+v)
+                        yp[u][v] = sp[b0 + u * sxs + 2 * ezp + v]; // (x+u, y+2, z+v)
+                        zm[u][v] = sp[b0 + u * sxs + v * ezp - 1]; // (x+u, y+v, z-1)
+                        zp[u][v] = sp[b0 + u * sxs + v * ezp + 2]; // (x+u, y+v, z+2)
                     }
                 float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
 #pragma unroll
