@@ -260,19 +260,42 @@ template <bool SMALL>
 __device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
                                                    const int e[3], float mass, unsigned long long& dep) {
     const int vol = e[0] * e[1] * e[2];
-    for (int i = threadIdx.x; i < vol; i += DEP_NT) win[i] = 0;
+    if constexpr (SMALL) {
+        // 4 cells per store; the window budget is a multiple of 4 cells
+        for (int i = 4 * threadIdx.x; i < vol; i += 4 * DEP_NT) *(uint4*)&win[i] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        for (int i = threadIdx.x; i < vol; i += DEP_NT) win[i] = 0;
+    }
     __syncthreads();
     dep += dep_accumulate<SMALL, true>(win, R, lo, e, 0, e[0], 0, e[1], mass);
 }
 // after a barrier that follows dep_one_accumulate
+// base is a multiple of 4 cells (reservations are): whole 4-cell groups move as 16 bytes.
 template <bool SMALL>
 __device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int e[3], unsigned* __restrict arena,
                                               unsigned long long base, unsigned long long& fl) {
     const int vol = e[0] * e[1] * e[2];
-    for (int i = threadIdx.x; i < vol; i += DEP_NT) {
-        const DepW<SMALL> v = win[i];
-        fl += (unsigned)v;
-        arena[base + i] = (unsigned)v;
+    if constexpr (SMALL) {
+        unsigned* __restrict out = arena + base;
+        for (int i = 4 * threadIdx.x; i < vol; i += 4 * DEP_NT) {
+            const uint4 v = *(const uint4*)&win[i];
+            if (i + 4 <= vol) {
+                fl += (unsigned long long)v.x + v.y + v.z + v.w;
+                *(uint4*)&out[i] = v;
+            } else {
+                const unsigned c[4] = {v.x, v.y, v.z, v.w};
+                for (int k = 0; i + k < vol; k++) {
+                    fl += c[k];
+                    out[i + k] = c[k];
+                }
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < vol; i += DEP_NT) {
+            const DepW<SMALL> v = win[i];
+            fl += (unsigned)v;
+            arena[base + i] = (unsigned)v;
+        }
     }
 }
 template <bool SMALL>
@@ -400,7 +423,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
                                                             unsigned long long* __restrict ctl,
                                                             unsigned long long* __restrict tile_total, int nsub,
                                                             int4* __restrict boxes, int ntb, int nblk) {
-    __shared__ unsigned long long win_raw[DEP_LDS / 8];
+    __shared__ __align__(16) unsigned long long win_raw[DEP_LDS / 8];
     DepW<SMALL>* win = (DepW<SMALL>*)win_raw;
     __shared__ int red[12][DEP_NT / 64];
     __shared__ unsigned long long s_tot[DEP_NT / 64], s_dif[DEP_NT / 64];
@@ -432,7 +455,8 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
         nc[d] = ((A[d] + e[d] - 1) >> 4) - c0[d] + 1;
     }
     const int nch = nc[0] * nc[1] * nc[2];
-    const unsigned long long V = (unsigned long long)e[0] * e[1] * e[2];
+    // arena reservation, a multiple of 4 cells so that every window starts 16-byte aligned
+    const unsigned long long V = ((unsigned long long)e[0] * e[1] * e[2] + 3ull) & ~3ull;
     // owner blocks of this tile's box (a batch holds several boxes of the same shape)
     const int blk0 = (tile / ntb) * nblk;
     // The sub-arena reservation and the chunk listing are issued first and their
@@ -452,7 +476,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     if (one) dep_one_accumulate<SMALL>(win, R, lo, e, mass, dep);
     if (fits) {
         if (tid == 0) {
-            const unsigned long long sub_cap = cap / nsub;
+            const unsigned long long sub_cap = (cap / nsub) & ~3ull;
             s_base = (unsigned long long)(tile % nsub) * sub_cap + rb;
             if (rb + V > sub_cap) s_full = 1;
         }
