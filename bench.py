@@ -40,11 +40,11 @@ def kernel_bytes(name, Np, G):
     table = {
         "drift": 44 * Np,                      # read pos 16 + vel 12, write pos 16
         "zero_density": 8 * G,                 # write int64 density
-        "deposit": 16 * Np + 8 * G,            # read pos (16 B packed), write density
+        "deposit": 16 * Np + 4 * G,            # read pos (16 B packed), write the FP32 density of the step
         "deposit_windows": 16 * Np,            # read pos (the window arena is overhead)
-        "deposit_owner": 8 * G,                # write density
+        "deposit_owner": 4 * G,                # write the FP32 density (the step path; int64 for hgpm_cic)
         # half spectra (k_z = 0..N/2) cost 4 B per grid cell
-        "fft_fwd_z_r2c": 12 * G,               # read int64, write the half spectrum
+        "fft_fwd_z_r2c": 8 * G,                # read the FP32 density, write the half spectrum
         "fft_fwd_y": 8 * G,                    # read + write one half spectrum
         "fft_x_psi": 8 * G,                    # read rho^, write psi^
         "fft_inv_y": 8 * G,                    # read + write one half spectrum
@@ -288,12 +288,12 @@ def main():
                  for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0])}
     # algorithmic bytes of one step of this algorithm: 72 per particle (deposit read of
     # the 16-byte packed position, gather/kick/drift read and write of position and
-    # velocity 56) and 56 per grid cell (density write 8, the five half-spectrum passes
-    # 12 + 8 + 8 + 8 + 8, potential read 4).  With 24-byte positions and a solve that
-    # transforms the three force fields a step moves 96 Np + 120 G with half spectra and
-    # 96 Np + 200 G with full complex spectra: the same step's rate against those byte
-    # counts is reported beside it.
-    step_alg = 72 * Np + 56 * G
+    # velocity 56) and 48 per grid cell (FP32 density write 4, the five half-spectrum
+    # passes 8 each, potential read 4).  With 24-byte positions, an int64 density and a
+    # solve that transforms the three force fields a step moves 96 Np + 120 G with half
+    # spectra and 96 Np + 200 G with full complex spectra: the same step's rate against
+    # those byte counts is reported beside it.
+    step_alg = 72 * Np + 48 * G
     step_r2c = 96 * Np + 120 * G
     step_c2c = 96 * Np + 200 * G
     out = {
