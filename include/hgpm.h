@@ -43,8 +43,9 @@ extern "C" {
 #define HGPM_MAX_STEPS 1000    /* MAX_STEPS, src/haccgpm.hpp:106 */
 
 /* precision of a context */
-#define HGPM_FP32 0  /* build layout: int32 cell + FP32 offset, int64 fixed-point deposit,
-                        complex64 Stockham FFT, fused kernels                          */
+#define HGPM_FP32 0  /* build layout: integer cell + FP32 offset (stored as one 32-bit
+                        cell index with 10 bits per axis, so NG <= 1024), int64
+                        fixed-point deposit, complex64 Stockham FFT, fused kernels     */
 #define HGPM_FP64 1  /* plain reference-structured path in FP64: double3 positions,
                         FP64 atomics, complex128 FFT, one kernel per reference kernel  */
 
