@@ -432,7 +432,7 @@ class MemoryManager:
 
     def autotune(self, reps=3):
         """Times the FFT variants on this grid and keeps the fastest; returns the
-        per-variant milliseconds (column pass, fused x-solve)."""
+        per-variant milliseconds (column pass, fused x pass of the solve)."""
         n = self.get_tunable("fft_variants")
         mc = np.zeros(n)
         mx = np.zeros(n)

@@ -1153,7 +1153,7 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
 }
 
 // Times every column-pass and x-pass variant of this grid size on the context's own
-// grids and keeps the fastest.  Overwrites the density / force grids.
+// grids and keeps the fastest.  Overwrites the density and the solve's grids.
 int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     if (!c) return HGPM_ERR_ARG;
     if (c->prec != HGPM_FP32) return HGPM_OK; // the FP64 plain path has one shape

@@ -440,7 +440,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     int lo[3], e[3], hb[2][6];
     dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
     if (tid == 0) {
-        // the gather of these positions stages its force window from these boxes
+        // the gather of these positions stages its potential window from these boxes
         boxes[4 * tile + 0] = make_int4(hb[0][0], hb[0][1], hb[0][2], hb[0][3]);
         boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
         boxes[4 * tile + 2] = make_int4(hb[1][0], hb[1][1], hb[1][2], hb[1][3]);

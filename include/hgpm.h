@@ -52,7 +52,7 @@ extern "C" {
 /* Mass unit of the int64 fixed-point density: one particle of mass (ng/np)^3 deposits
    round(w * 2^24) per cell, w the FP32 CIC weight times the mass. */
 #define HGPM_MASS_BITS 24
-#define HGPM_FFT_VARIANTS 6 /* FFT column-pass / x-solve shapes, see hgpm_set_tunable */
+#define HGPM_FFT_VARIANTS 6 /* FFT column-pass / x-pass shapes, see hgpm_set_tunable */
 
 /* HACCGPM::Params (src/haccgpm.hpp:118-174), the fields the serial path reads. */
 typedef struct hgpm_params {
@@ -236,7 +236,7 @@ int hgpm_get_timings(hgpm_ctx* ctx, char* names, int names_len, double* total_ms
                      int64_t* launches, int max, int* n);
 int hgpm_reset_timings(hgpm_ctx* ctx);
 /* Tunables: "fft_cols" and "fft_xsolve" select the FFT column-pass and fused
-   x-solve shapes (columns per tile x threads: 0 = 4x256, 1 = 4x512, 2 = 8x512,
+   x-pass shapes (columns per tile x threads: 0 = 4x256, 1 = 4x512, 2 = 8x512,
    3 = 16x512, 4 = 8x1024, 5 = 16x1024, where the tile fits LDS); "fft_variants" (get
    only) is the number of variant ids (HGPM_FFT_VARIANTS) and "fft_variant_mask" (get
    only) has bit v set when variant v is available for this grid.  Deposit routing (FP32): "deposit_arena" = usable cells of the window
@@ -251,7 +251,7 @@ int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
    FFT variant on this context's grids (reps launches each), keeps the fastest, and
    reports per-variant ms in ms_cols / ms_xsolve (HGPM_FFT_VARIANTS entries each, -1
    for variants not available on this grid; may be NULL).
-   Clobbers the density and force grids. */
+   Clobbers the density and the solve's grids. */
 int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
 
 /* ------------------------------------------------------------- ensembles ---- */

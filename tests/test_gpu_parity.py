@@ -163,7 +163,7 @@ def test_solve_fp32_vs_fp64_oracle(hg, orc, ng):
 
 @pytest.mark.parametrize("ng", [32, 96, 128])
 def test_fft_variants_agree(hg, orc, ng):
-    """Every column-pass / x-solve variant the autotuner may pick gives the same physics."""
+    """Every column-pass / x-pass variant the autotuner may pick gives the same physics."""
     case(f"fft variants ng={ng}")
     rng = np.random.default_rng(ng + 21)
     with mk(hg, ng) as m:
