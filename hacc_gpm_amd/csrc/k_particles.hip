@@ -797,6 +797,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
         const int nbx = min(tx, b[0] - sx0), nxs = nbx + GT_HALO;
         for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
             const int nby = min(ty, b[1] - sy0), nys = nby + GT_HALO;
+            int gxs, gys, gzs;
             {
                 // Staging in 4-cell segments of the slab's rows (row = staged (x, y), ezp
                 // cells of z): item it -> row it / nseg, segment it % nseg.  Cells past ez
@@ -806,6 +807,10 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 const int wz0 = wrap_cell(bz0, ng);
                 // the padded row inside one grid row (uniform per workgroup): no z wrap
                 const bool zin = wz0 + ezp <= ng;
+                // first base cell of the slab, wrapped (for the particles' offsets below)
+                gxs = wrap_cell(bx0 + 1, ng);
+                gys = wrap_cell(by0 + 1, ng);
+                gzs = wrap_cell(bz0 + 1, ng);
                 const int nitems = nxs * nys * nseg;
                 // it -> (row, segment) and row -> (x, y) by multiplication with rounded
                 // reciprocals, each quotient corrected by one step (exact for any sizes here)
@@ -859,10 +864,16 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 // GT_PPT particles from being hoisted out of the slab loops (registers)
                 PosSplit p = P[k];
                 asm volatile("" : "+v"(p.cx), "+v"(p.cy), "+v"(p.cz), "+v"(p.fx), "+v"(p.fy), "+v"(p.fz));
-                const int lx = wrap_rel(p.cx - r0.cx, ng) - lo[0] - sx0;
-                const int ly = wrap_rel(p.cy - r0.cy, ng) - lo[1] - sy0;
-                if (lx < 0 || lx >= nbx || ly < 0 || ly >= nby) continue;
-                const int lz = wrap_rel(p.cz - r0.cz, ng) - lo[2];
+                // base cell relative to the slab's first base cell, modulo ng: the true
+                // offset lies in [0, nbx) for the slab's particles, and every other
+                // particle of the tile lands at >= nbx (its offset is in [-sx0, b - sx0)
+                // with sx0 + nbx <= b <= ng); likewise in y, and in z every particle is
+                // inside [0, b[2])
+                int lx = p.cx - gxs, ly = p.cy - gys, lz = p.cz - gzs;
+                lx += (lx >> 31) & ng;
+                ly += (ly >> 31) & ng;
+                if ((unsigned)lx >= (unsigned)nbx || (unsigned)ly >= (unsigned)nby) continue;
+                lz += (lz >> 31) & ng;
                 const float gx[2] = {1.0f - p.fx, p.fx};
                 const float gy[2] = {1.0f - p.fy, p.fy};
                 const float gz[2] = {1.0f - p.fz, p.fz};
