@@ -136,6 +136,8 @@ def lib():
             "hgpm_ensemble_create": (c_int, [c_int, vp, P(vp)]),
             "hgpm_ensemble_destroy": (c_int, [vp]),
             "hgpm_ensemble_last_error": (c_char_p, [vp]),
+            "hgpm_ensemble_create_budget": (c_int, [c_int, vp, ctypes.c_size_t, P(vp)]),
+            "hgpm_ensemble_waves": (c_int, [vp, vp]),
             "hgpm_ensemble_batches": (c_int, [vp, P(c_int), vp]),
             "hgpm_ensemble_device_bytes": (c_int, [vp, P(ctypes.c_size_t)]),
             "hgpm_ensemble_generate_ic": (c_int, [vp, c_int, vp, c_int, c_double, c_double, c_double, c_double,
@@ -463,11 +465,13 @@ class Ensemble:
     """Many boxes stepped together (hgpm_ensemble_*): boxes with the same dynamics are
     batched into one set of launches per step; seeds and spectra stay per box."""
 
-    def __init__(self, params_list):
+    def __init__(self, params_list, device_budget=0):
+        """device_budget: bytes the ensemble may use (0: 90% of the free device memory);
+        batches that do not fit keep grid buffers for a wave of their boxes only."""
         self.params = list(params_list)
         arr = (Params * len(self.params))(*self.params)
         h = ctypes.c_void_p()
-        rc = lib().hgpm_ensemble_create(len(self.params), arr, ctypes.byref(h))
+        rc = lib().hgpm_ensemble_create_budget(len(self.params), arr, int(device_budget), ctypes.byref(h))
         if rc:
             raise HgpmError(rc, lib().hgpm_last_error(None).decode())
         self.h = h
@@ -498,6 +502,13 @@ class Ensemble:
         of = np.zeros(len(self.params), np.int32)
         self._c(lib().hgpm_ensemble_batches(self.h, ctypes.byref(n), _ptr(of)))
         return n.value, of
+
+    def waves(self):
+        """Boxes per wave of each batch (a batch's size when it is fully resident)."""
+        n, _ = self.batches()
+        w = np.zeros(n, np.int32)
+        self._c(lib().hgpm_ensemble_waves(self.h, _ptr(w)))
+        return w
 
     def device_bytes(self):
         b = ctypes.c_size_t()

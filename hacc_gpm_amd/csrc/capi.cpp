@@ -33,7 +33,12 @@ struct hgpm_ctx {
     hgpm_params p{};
     int ng = 0, np = 0, prec = HGPM_FP32;
     int64_t N = 0, G = 0, Gic = 0; // per box
-    int nbox = 1; // FP32 batches: boxes of the same shape stored back to back
+    int nbox = 1; // FP32 batches: boxes of the same shape stored back to back (this launch)
+    // Boxes whose particles are stored (nstore) and boxes the grids, arena and tile
+    // buffers hold (nwave).  nstore > nwave: a step runs the boxes in waves of nwave,
+    // one after the other, reusing the grid buffers (their contents do not outlive a
+    // step); only the particles persist for every box.
+    int nstore = 1, nwave = 1;
     hipStream_t st = nullptr;
     int device = 0;
     std::string err;
@@ -387,7 +392,7 @@ const char* hgpm_version(void) { return "hgpm 0.1.0 (gfx950)"; }
 const char* hgpm_last_error(const hgpm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
 
 namespace {
-int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
+int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
     if (!p || !out) return fail(nullptr, HGPM_ERR_ARG, "hgpm_create: null argument");
     if (nbox < 1 || (nbox > 1 && p->precision != HGPM_FP32))
         return fail(nullptr, HGPM_ERR_ARG, "batches of boxes need FP32 and at least one box");
@@ -416,7 +421,10 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
     c->N = (int64_t)p->np * p->np * p->np;
     c->G = (int64_t)p->ng * p->ng * p->ng;
     c->Gic = c->G; // np <= ng
-    c->nbox = nbox;
+    if (nwave < 1 || nwave > nbox) nwave = nbox;
+    c->nstore = nbox;
+    c->nwave = nwave;
+    c->nbox = nwave;
     c->fft = fft_ops(p->ng);
     c->fftic = fft_ops(p->np);
     (void)hipGetDevice(&c->device);
@@ -433,8 +441,10 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out) {
     }
     const size_t N = (size_t)c->N, G = (size_t)c->G;
     if (c->prec == HGPM_FP32) {
-        // every per-box buffer holds nbox boxes back to back
-        const size_t NB = N * nbox, GB = G * nbox;
+        // particles for every stored box; every other per-box buffer for one wave
+        const size_t NB = N * (size_t)c->nstore;
+        nbox = nwave;
+        const size_t GB = G * nbox;
         if ((rc = dalloc(c, &c->pos, sizeof(PosPackH) * NB, "pos")) ||
             (rc = dalloc(c, &c->vel, sizeof(float) * 3 * NB, "vel")) ||
             (rc = dalloc(c, &c->A, sizeof(float2) * GB, "grid A")) ||
@@ -932,14 +942,47 @@ int hgpm_kick_drift(hgpm_ctx* c, double deltaT, double fscal, double prefactor) 
     return hgpm_update_positions(c, prefactor);
 }
 
-int hgpm_step(hgpm_ctx* c, const hgpm_step_coeffs* k) {
-    if (!c || !k) return fail(c, HGPM_ERR_ARG, "hgpm_step: null argument");
+} // extern "C"
+namespace {
+int step_impl(hgpm_ctx* c, const hgpm_step_coeffs* k) {
     int rc = need_particles(c);
     if (rc) return rc;
     if (!k->skip_drift0 && (rc = hgpm_update_positions(c, k->drift0))) return rc;
     if ((rc = cic_impl(c, true))) return rc;
     if ((rc = hgpm_solve_gradient(c))) return rc;
     return hgpm_kick_drift(c, k->kick_dt, k->kick_fscal, k->drift1);
+}
+// Boxes [first, first + count) of a context stored in waves, seen as a batch of count
+// boxes: the particles moved to their slice; grids, arena and tile buffers are the
+// wave buffers.  Restored on destruction.
+struct WaveView {
+    hgpm_ctx* c;
+    PosPackH* pos;
+    float* vel;
+    int nbox;
+    WaveView(hgpm_ctx* c_, int first, int count) : c(c_), pos(c_->pos), vel(c_->vel), nbox(c_->nbox) {
+        c->pos += (int64_t)first * c->N;
+        c->vel += 3 * (int64_t)first * c->N;
+        c->nbox = count;
+    }
+    ~WaveView() {
+        c->pos = pos; c->vel = vel; c->nbox = nbox;
+        c->boxes_valid = false;
+    }
+};
+} // namespace
+extern "C" {
+
+int hgpm_step(hgpm_ctx* c, const hgpm_step_coeffs* k) {
+    if (!c || !k) return fail(c, HGPM_ERR_ARG, "hgpm_step: null argument");
+    if (c->nstore <= c->nwave) return step_impl(c, k);
+    // boxes stored in waves: each wave steps through the shared grid buffers in turn
+    for (int first = 0; first < c->nstore; first += c->nwave) {
+        WaveView v(c, first, std::min(c->nwave, c->nstore - first));
+        int rc = step_impl(c, k);
+        if (rc) return rc;
+    }
+    return HGPM_OK;
 }
 
 // ------------------------------------------------------------------- FFT ----
@@ -1218,36 +1261,44 @@ struct hgpm_ensemble {
 };
 
 namespace {
-// Box b of a batch seen as a single-box context: the per-box buffers moved to its
-// slice, its own parameters (seed, spectrum), nbox = 1; restored on destruction.  A
-// deposit made through the view rewrites the shared cell boxes of the gather, which
-// are then invalid for the batch.
+// Box b of a batch seen as a single-box context: the particles moved to its slice,
+// its own parameters (seed, spectrum), nbox = 1; the grids, arena and tile buffers
+// are used from their start (their contents do not outlive one operation).  Restored
+// on destruction.  A deposit made through the view rewrites the shared cell boxes of
+// the gather, which are then invalid for the batch.
 struct BoxView {
     hgpm_ctx* c;
     hgpm_params p;
     PosPackH* pos;
-    float *vel, *psi;
-    float2 *A, *B;
-    unsigned long long* total;
+    float* vel;
     int nbox;
     BoxView(hgpm_ctx* c_, int b, const hgpm_params& bp) : c(c_) {
-        p = c->p; pos = c->pos; vel = c->vel; psi = c->psi; A = c->A; B = c->B;
-        total = c->total; nbox = c->nbox;
+        p = c->p; pos = c->pos; vel = c->vel; nbox = c->nbox;
         c->p = bp;
         c->pos += (int64_t)b * c->N;
         c->vel += 3 * (int64_t)b * c->N;
-        c->psi += (int64_t)b * c->G;
-        c->A += (int64_t)b * c->G;
-        c->B += (int64_t)b * c->G;
-        c->total += b;
         c->nbox = 1;
     }
     ~BoxView() {
-        c->p = p; c->pos = pos; c->vel = vel; c->psi = psi; c->A = A; c->B = B;
-        c->total = total; c->nbox = nbox;
+        c->p = p; c->pos = pos; c->vel = vel; c->nbox = nbox;
         c->boxes_valid = false;
     }
 };
+
+// Device bytes of a batch context of nstore boxes with grid buffers for nwave (the
+// allocations of ctx_create, FP32).
+size_t batch_bytes(const hgpm_params& p, int nstore, int nwave) {
+    const size_t N = (size_t)p.np * p.np * p.np, G = (size_t)p.ng * p.ng * p.ng;
+    const size_t nblk = (size_t)(p.ng / 16) * (p.ng / 16) * (p.ng / 16) * nwave;
+    const size_t ntiles = N * nwave / (16 * 16 * 16);
+    size_t b = (sizeof(PosPackH) + 3 * sizeof(float)) * N * nstore;     // particles
+    b += (2 * sizeof(float2) + sizeof(float)) * G * nwave;              // A, B, psi
+    b += 6 * sizeof(unsigned) * G * nwave;                              // arena
+    b += (16 * HGPM_DEP_SLOTS + sizeof(unsigned)) * nblk;               // slots, counts
+    b += (sizeof(unsigned) + sizeof(unsigned long long) + 64) * ntiles; // tile buffers
+    b += 64ull << 20;                                                   // tables, P(k), slack
+    return b;
+}
 
 bool same_dynamics(const hgpm_params& a, const hgpm_params& b) {
     return a.ng == b.ng && a.np == b.np && a.rl == b.rl && a.z_ini == b.z_ini && a.z_fin == b.z_fin &&
@@ -1279,6 +1330,11 @@ int eforward(hgpm_ensemble* e, hgpm_ctx* c, int rc) {
 extern "C" {
 
 int hgpm_ensemble_create(int n_boxes, const hgpm_params* params, hgpm_ensemble** out) {
+    return hgpm_ensemble_create_budget(n_boxes, params, 0, out);
+}
+
+int hgpm_ensemble_create_budget(int n_boxes, const hgpm_params* params, size_t device_budget,
+                                hgpm_ensemble** out) {
     if (!params || !out || n_boxes < 1) return efail(nullptr, HGPM_ERR_ARG, "hgpm_ensemble_create: bad arguments");
     *out = nullptr;
     hgpm_ensemble* e = new hgpm_ensemble();
@@ -1302,9 +1358,47 @@ int hgpm_ensemble_create(int n_boxes, const hgpm_params* params, hgpm_ensemble**
         e->slot_of[b] = (int)members[g].size();
         members[g].push_back(b);
     }
-    for (auto& m : members) {
+    // Waves: while the batches do not fit the budget, halve the wave of the batch whose
+    // grid buffers would shrink the most (particles always stay resident).
+    if (device_budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            delete e;
+            return efail(nullptr, HGPM_ERR_HIP, "hipMemGetInfo failed");
+        }
+        device_budget = free_b / 10 * 9;
+    }
+    std::vector<int> wave(members.size());
+    for (size_t g = 0; g < members.size(); g++) wave[g] = (int)members[g].size();
+    auto total_bytes = [&]() {
+        size_t t = 0;
+        for (size_t g = 0; g < members.size(); g++) t += batch_bytes(params[members[g][0]], (int)members[g].size(), wave[g]);
+        return t;
+    };
+    while (total_bytes() > device_budget) {
+        int best = -1;
+        size_t gain = 0;
+        for (size_t g = 0; g < members.size(); g++) {
+            if (wave[g] <= 1) continue;
+            const hgpm_params& q = params[members[g][0]];
+            const int w2 = (wave[g] + 1) / 2;
+            const size_t d = batch_bytes(q, (int)members[g].size(), wave[g]) - batch_bytes(q, (int)members[g].size(), w2);
+            if (d > gain) { gain = d; best = (int)g; }
+        }
+        if (best < 0) {
+            delete e;
+            return efail(nullptr, HGPM_ERR_ALLOC,
+                         "ensemble does not fit the device budget even one box per wave (" +
+                             std::to_string(total_bytes() >> 20) + " MiB > " + std::to_string(device_budget >> 20) +
+                             " MiB)");
+        }
+        wave[best] = (wave[best] + 1) / 2;
+    }
+    for (size_t g = 0; g < members.size(); g++) {
+        auto& m = members[g];
         hgpm_ctx* c = nullptr;
-        int rc = ctx_create(&params[m[0]], (int)m.size(), &c);
+        int rc = ctx_create(&params[m[0]], (int)m.size(), &c, wave[g]);
         if (rc != HGPM_OK) {
             for (hgpm_ctx* q : e->batch) hgpm_destroy(q);
             delete e;
@@ -1331,6 +1425,12 @@ int hgpm_ensemble_batches(const hgpm_ensemble* e, int* n_batches, int* batch_of_
     *n_batches = (int)e->batch.size();
     if (batch_of_box)
         for (size_t b = 0; b < e->batch_of.size(); b++) batch_of_box[b] = e->batch_of[b];
+    return HGPM_OK;
+}
+
+int hgpm_ensemble_waves(const hgpm_ensemble* e, int* boxes_per_wave) {
+    if (!e || !boxes_per_wave) return HGPM_ERR_ARG;
+    for (size_t g = 0; g < e->batch.size(); g++) boxes_per_wave[g] = e->batch[g]->nwave;
     return HGPM_OK;
 }
 

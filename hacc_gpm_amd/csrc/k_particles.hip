@@ -440,10 +440,21 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     int lo[3], e[3], hb[2][6];
     dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
     if (tid == 0) {
-        // the gather of these positions stages its potential window from these boxes
-        boxes[4 * tile + 0] = make_int4(hb[0][0], hb[0][1], hb[0][2], hb[0][3]);
+        // the gather of these positions stages its potential window from these boxes.
+        // Their origins are stored as absolute cells (centred into [-ng/2, ng/2)): the
+        // gather must not re-read r0, whose particle another gather workgroup drifts.
+        int o[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int rc = d == 0 ? r0.cx : (d == 1 ? r0.cy : r0.cz);
+                int a = wrap_cell(rc + hb[h][d], ng);
+                o[h][d] = a >= ng / 2 ? a - ng : a;
+            }
+        boxes[4 * tile + 0] = make_int4(o[0][0], o[0][1], o[0][2], hb[0][3]);
         boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
-        boxes[4 * tile + 2] = make_int4(hb[1][0], hb[1][1], hb[1][2], hb[1][3]);
+        boxes[4 * tile + 2] = make_int4(o[1][0], o[1][1], o[1][2], hb[1][3]);
         boxes[4 * tile + 3] = make_int4(hb[1][4], hb[1][5], 0, 0);
     }
     const int nb = ng / OB;
@@ -731,10 +742,11 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
     // the potential of this tile's box (a batch holds several boxes of the same shape)
     psi += (p0 / npb) * gcells;
-    // BOXED: the deposit of these positions stored the box of each half deposit tile,
-    // relative to the deposit tile's first particle; no reduction, and the staging
-    // loads can start while the particle loads are in flight
-    const PosSplit r0 = unpack(pos[BOXED ? p0 / DEP_TILE * DEP_TILE : p0]);
+    // BOXED: the deposit of these positions stored the box of each half deposit tile
+    // with an absolute origin; no reduction, and the staging loads can start while the
+    // particle loads are in flight.  Otherwise the box is reduced relative to this
+    // workgroup's own first particle (read before this workgroup writes it).
+    const PosSplit r0 = BOXED ? PosSplit{0, 0, 0, 0.0f, 0.0f, 0.0f} : unpack(pos[p0]);
     PosSplit P[GT_PPT];
     float V[GT_PPT][3];
     int lo[3], b[3];

@@ -261,7 +261,18 @@ int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
    covers in one launch; batches run on their own streams.  Seeds and spectra are per
    box.  FP32 engine only. */
 typedef struct hgpm_ensemble hgpm_ensemble;
+/* Uses at most 90% of the device memory free at the call (hgpm_ensemble_create_budget). */
 int hgpm_ensemble_create(int n_boxes, const hgpm_params* params, hgpm_ensemble** out);
+/* Memory-pressure scheduling: when the batches, every box fully resident, exceed
+   device_budget bytes (0: 90% of the free device memory), the particles of every box
+   stay resident but a batch keeps grid, arena and tile buffers for only a wave of its
+   boxes, and a step runs the waves one after the other (the wave of the batch whose
+   buffers shrink most is halved until the total fits).  Results are bit-identical to
+   the fully resident ensemble.  HGPM_ERR_ALLOC if even one box per wave does not fit. */
+int hgpm_ensemble_create_budget(int n_boxes, const hgpm_params* params, size_t device_budget,
+                                hgpm_ensemble** out);
+/* Boxes per wave of each batch (n_batches entries). */
+int hgpm_ensemble_waves(const hgpm_ensemble* e, int* boxes_per_wave);
 int hgpm_ensemble_destroy(hgpm_ensemble* e);
 const char* hgpm_ensemble_last_error(const hgpm_ensemble* e);
 /* Number of batches formed; batch_of_box (n_boxes entries, may be NULL) maps boxes. */

@@ -534,3 +534,34 @@ def test_ensemble_matches_single_boxes(hg):
         finally:
             for m in singles:
                 m.close()
+
+
+
+def test_ensemble_waves_match_resident(hg):
+    """An ensemble squeezed into a small device budget keeps every box's particles but
+    grid buffers for a wave of boxes only, and steps the waves in turn: bit-identical
+    to the same ensemble fully resident.  A budget below one box per wave is refused."""
+    case("ensemble 6 x 64^3 in waves")
+    ps = [hg.make_params(64, 64, 128.0, 30.0, 0.0, 40, seed=20 + b, ss8=0.75 + 0.02 * b) for b in range(6)]
+    n_steps = 4
+    with hg.Ensemble(ps) as full, hg.Ensemble(ps, device_budget=150 << 20) as waved:
+        assert full.waves()[0] == 6
+        w = int(waved.waves()[0])
+        assert 1 <= w < 6, w
+        assert waved.device_bytes() < full.device_bytes()
+        scheds = []
+        for b, p in enumerate(ps):
+            hdr, vals, delta, dot_delta, ts = _ic_inputs(hg, p)
+            full.generate_ic(b, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+            waved.generate_ic(b, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+            scheds.append(hg.schedule(p, n_steps, merge=True))
+        for s in range(n_steps):
+            full.step([scheds[b][s] for b in range(len(ps))])
+            waved.step([scheds[b][s] for b in range(len(ps))])
+        for b in range(len(ps)):
+            p1, v1 = full.get_particles(b)
+            p2, v2 = waved.get_particles(b)
+            assert np.array_equal(p1, p2) and np.array_equal(v1, v2), b
+            assert np.array_equal(full.power_spectrum(b)[1], waved.power_spectrum(b)[1]), b
+    with pytest.raises(hg.HgpmError):
+        hg.Ensemble(ps, device_budget=16 << 20)
