@@ -168,8 +168,9 @@ def ic_args(hg, p):
 
 
 def run_ensemble(hg, steps, warmup, per_class):
-    """Batched ensemble vs the same boxes launched one at a time (one context per box,
-    each box's step completed before the next box's starts)."""
+    """Batched ensemble, its steps captured into one HIP graph, vs the same boxes
+    launched one at a time (one context per box, each box's step completed before the
+    next box's starts).  The batched run with eager launches is reported beside it."""
     ps = ensemble_boxes(hg, per_class)
     n_total = warmup + steps
     ics = [ic_args(hg, p) for p in ps]
@@ -186,9 +187,21 @@ def run_ensemble(hg, steps, warmup, per_class):
         for s in range(warmup, n_total):
             e.step([sc[s] for sc in scheds])
         e.synchronize()
-        t_batched = time.perf_counter() - t0
+        t_eager = time.perf_counter() - t0
         nbatches, _ = e.batches()
         dev = e.device_bytes()
+    # the same run with the steps captured into one HIP graph (capture and instantiation
+    # inside the timed region)
+    with hg.Ensemble(ps) as e:
+        e.autotune(2)
+        for b, a in enumerate(ics):
+            e.generate_ic(b, *a)
+        e.run([[sc[s] for sc in scheds] for s in range(warmup)])
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.run([[sc[s] for sc in scheds] for s in range(warmup, n_total)])
+        e.synchronize()
+        t_batched = time.perf_counter() - t0
     t_single = 0.0
     for b, p in enumerate(ps):
         with hg.MemoryManager(p) as m:
@@ -202,7 +215,7 @@ def run_ensemble(hg, steps, warmup, per_class):
                 m.synchronize()
             t_single += time.perf_counter() - t0
     # each size class batched on its own (no other class sharing the GPU)
-    per_class = []
+    classes = []
     for n in sorted({p.np for p in ps}):
         idx = [b for b, p in enumerate(ps) if p.np == n]
         with hg.Ensemble([ps[b] for b in idx]) as e:
@@ -217,14 +230,15 @@ def run_ensemble(hg, steps, warmup, per_class):
                 e.step([scheds[b][s] for b in idx])
             e.synchronize()
             t = time.perf_counter() - t0
-        per_class.append({"np": n, "boxes": len(idx), "ms_per_step": t * 1e3 / steps,
+        classes.append({"np": n, "boxes": len(idx), "ms_per_step": t * 1e3 / steps,
                           "particle_steps_per_s": len(idx) * n ** 3 * steps / t})
     return {"metric": "aggregate particle-steps/s, batched ensemble vs one box at a time",
             "value": nparts * steps / t_batched, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
             "warmup": warmup, "ms_per_step": t_batched * 1e3 / steps, "higher_is_better": True,
             "one_at_a_time": {"value": nparts * steps / t_single, "ms_per_step": t_single * 1e3 / steps},
+            "batched_eager": {"value": nparts * steps / t_eager, "ms_per_step": t_eager * 1e3 / steps},
             "speedup": t_single / t_batched, "batches": nbatches, "device_GB": dev / 1e9,
-            "per_class_batched_alone": per_class,
+            "per_class_batched_alone": classes,
             "dtype": "f32", "data": "synthetic: Zel'dovich ICs, Eisenstein-Hu no-wiggle P(k), per-box seed and sigma_8",
             "config": {"workload": f"ensemble_{per_class}x(64^3,128^3,192^3)_box2Mpc_per_cell_z30",
                        "boxes": len(ps), "particles": nparts}}

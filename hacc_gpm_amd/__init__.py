@@ -146,6 +146,7 @@ def lib():
             "hgpm_ensemble_get_particles": (c_int, [vp, c_int, vp, vp]),
             "hgpm_ensemble_power_spectrum": (c_int, [vp, c_int, c_int, vp, vp, vp]),
             "hgpm_ensemble_step": (c_int, [vp, vp]),
+            "hgpm_ensemble_run": (c_int, [vp, c_int, vp]),
             "hgpm_ensemble_synchronize": (c_int, [vp]),
             "hgpm_ensemble_autotune": (c_int, [vp, c_int]),
         }
@@ -545,6 +546,15 @@ class Ensemble:
         """coeffs: one StepCoeffs per box (equal within a batch)."""
         arr = (StepCoeffs * len(self.params))(*coeffs)
         self._c(lib().hgpm_ensemble_step(self.h, arr))
+
+    def run(self, coeffs_per_step):
+        """Several steps as one HIP graph: coeffs_per_step[s] is the per-box list
+        step() takes for step s."""
+        steps = [list(c) for c in coeffs_per_step]
+        nb = len(self.params)
+        assert all(len(c) == nb for c in steps)
+        arr = (StepCoeffs * (nb * len(steps)))(*[q for c in steps for q in c])
+        self._c(lib().hgpm_ensemble_run(self.h, len(steps), arr))
 
     def synchronize(self):
         self._c(lib().hgpm_ensemble_synchronize(self.h))

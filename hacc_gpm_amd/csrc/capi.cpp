@@ -1258,9 +1258,38 @@ struct hgpm_ensemble {
     std::vector<int> first;             // per batch: its first box
     std::vector<hgpm_params> params;    // per box
     std::string err;
+    // hgpm_ensemble_run: one event per batch stream (fork / join / done) and the
+    // launched graphs, each with the event that marks its end, freed once it passed.
+    std::vector<hipEvent_t> ev;
+    std::vector<std::pair<hipGraphExec_t, hipEvent_t>> graphs;
 };
 
 namespace {
+// Destroys the graphs of hgpm_ensemble_run that have finished (all of them if all: the
+// caller has synchronized every stream).
+void release_graphs(hgpm_ensemble* e, bool all) {
+    size_t k = 0;
+    for (auto& g : e->graphs) {
+        if (all || hipEventQuery(g.second) == hipSuccess) {
+            (void)hipGraphExecDestroy(g.first);
+            (void)hipEventDestroy(g.second);
+        } else {
+            e->graphs[k++] = g;
+        }
+    }
+    e->graphs.resize(k);
+}
+// coeffs[b] must agree within each batch.
+int check_coeffs(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs) {
+    for (size_t b = 0; b < e->params.size(); b++) {
+        const hgpm_step_coeffs& a = coeffs[e->first[e->batch_of[b]]];
+        const hgpm_step_coeffs& q = coeffs[b];
+        if (a.drift0 != q.drift0 || a.kick_dt != q.kick_dt || a.kick_fscal != q.kick_fscal || a.drift1 != q.drift1 ||
+            a.skip_drift0 != q.skip_drift0)
+            return HGPM_ERR_ARG;
+    }
+    return HGPM_OK;
+}
 // Box b of a batch seen as a single-box context: the particles moved to its slice,
 // its own parameters (seed, spectrum), nbox = 1; the grids, arena and tile buffers
 // are used from their start (their contents do not outlive one operation).  Restored
@@ -1413,6 +1442,9 @@ int hgpm_ensemble_create_budget(int n_boxes, const hgpm_params* params, size_t d
 
 int hgpm_ensemble_destroy(hgpm_ensemble* e) {
     if (!e) return HGPM_OK;
+    for (hgpm_ctx* c : e->batch) (void)hipStreamSynchronize(c->st);
+    release_graphs(e, true);
+    for (hipEvent_t v : e->ev) (void)hipEventDestroy(v);
     for (hgpm_ctx* c : e->batch) hgpm_destroy(c);
     delete e;
     return HGPM_OK;
@@ -1479,17 +1511,95 @@ int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k,
 // coefficients (their timesteppers agree), so coeffs must agree within a batch.
 int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs) {
     if (!e || !coeffs) return efail(e, HGPM_ERR_ARG, "hgpm_ensemble_step: null argument");
-    for (size_t b = 0; b < e->params.size(); b++) {
-        const hgpm_step_coeffs& a = coeffs[e->first[e->batch_of[b]]];
-        const hgpm_step_coeffs& q = coeffs[b];
-        if (a.drift0 != q.drift0 || a.kick_dt != q.kick_dt || a.kick_fscal != q.kick_fscal || a.drift1 != q.drift1 ||
-            a.skip_drift0 != q.skip_drift0)
-            return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch (box " + std::to_string(b) + ")");
-    }
+    if (check_coeffs(e, coeffs)) return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch");
     for (size_t g = 0; g < e->batch.size(); g++) {
         int rc = hgpm_step(e->batch[g], &coeffs[e->first[g]]);
         if (rc) return eforward(e, e->batch[g], rc);
     }
+    return HGPM_OK;
+}
+
+// nsteps steps of every box as one HIP graph: the launches of every batch stream are
+// captured (stream 0 forks to the other batch streams and joins them at the end),
+// instantiated and launched once on stream 0.  The launches and their arguments are
+// those hgpm_ensemble_step would make, so the results are bit-identical.  Per-kernel
+// timing is off inside the graph.  The graph holds no memset node: buffers are cleared
+// by fill kernels.
+int hgpm_ensemble_run(hgpm_ensemble* e, int nsteps, const hgpm_step_coeffs* coeffs) {
+    if (!e || (!coeffs && nsteps > 0) || nsteps < 0) return efail(e, HGPM_ERR_ARG, "hgpm_ensemble_run: bad arguments");
+    const size_t nb = e->params.size(), ng = e->batch.size();
+    for (int s = 0; s < nsteps; s++)
+        if (check_coeffs(e, coeffs + (size_t)s * nb))
+            return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch (step " + std::to_string(s) + ")");
+    for (hgpm_ctx* c : e->batch) {
+        int rc = need_particles(c);
+        if (rc) return eforward(e, c, rc);
+    }
+    if (nsteps == 0) return HGPM_OK;
+    release_graphs(e, false);
+    while (e->ev.size() < ng) {
+        hipEvent_t v;
+        if (hipEventCreateWithFlags(&v, hipEventDisableTiming) != hipSuccess)
+            return efail(e, HGPM_ERR_HIP, "hgpm_ensemble_run: hipEventCreate failed");
+        e->ev.push_back(v);
+    }
+    hipStream_t s0 = e->batch[0]->st;
+    // the graph runs after the work already queued on every batch stream
+    for (size_t g = 1; g < ng; g++) {
+        if (hipEventRecord(e->ev[g], e->batch[g]->st) != hipSuccess || hipStreamWaitEvent(s0, e->ev[g], 0) != hipSuccess)
+            return efail(e, HGPM_ERR_HIP, "hgpm_ensemble_run: stream ordering failed");
+    }
+    std::vector<int> timing(ng);
+    for (size_t g = 0; g < ng; g++) { timing[g] = e->batch[g]->timing; e->batch[g]->timing = false; }
+    std::string what;
+    int rc = HGPM_OK;
+    hipError_t he = hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal);
+    if (he != hipSuccess) { what = "hipStreamBeginCapture"; rc = HGPM_ERR_HIP; }
+    if (!rc && hipEventRecord(e->ev[0], s0) != hipSuccess) { what = "fork"; rc = HGPM_ERR_HIP; }
+    size_t joined = 1;
+    for (; !rc && joined < ng; joined++)
+        if (hipStreamWaitEvent(e->batch[joined]->st, e->ev[0], 0) != hipSuccess) { what = "fork"; rc = HGPM_ERR_HIP; }
+    if (rc) joined--;
+    for (int s = 0; !rc && s < nsteps; s++)
+        for (size_t g = 0; !rc && g < ng; g++)
+            if ((rc = hgpm_step(e->batch[g], &coeffs[(size_t)s * nb + e->first[g]]))) what = e->batch[g]->err;
+    // join every stream that took part, also after a failure, so the capture ends cleanly
+    for (size_t g = 1; g < joined && he == hipSuccess; g++) {
+        (void)hipEventRecord(e->ev[g], e->batch[g]->st);
+        (void)hipStreamWaitEvent(s0, e->ev[g], 0);
+    }
+    hipGraph_t graph = nullptr;
+    if (he == hipSuccess && hipStreamEndCapture(s0, &graph) != hipSuccess && !rc) {
+        what = "hipStreamEndCapture";
+        rc = HGPM_ERR_HIP;
+    }
+    for (size_t g = 0; g < ng; g++) e->batch[g]->timing = timing[g] != 0;
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        what = "hipGraphInstantiate";
+        rc = HGPM_ERR_HIP;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        return efail(e, rc, "hgpm_ensemble_run: " + what);
+    }
+    hipEvent_t done;
+    if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        return efail(e, HGPM_ERR_HIP, "hgpm_ensemble_run: hipEventCreate failed");
+    }
+    if (hipGraphLaunch(exec, s0) != hipSuccess || hipEventRecord(done, s0) != hipSuccess) {
+        (void)hipStreamSynchronize(s0);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipEventDestroy(done);
+        return efail(e, HGPM_ERR_HIP, "hgpm_ensemble_run: hipGraphLaunch failed");
+    }
+    e->graphs.emplace_back(exec, done);
+    // later work on the other batch streams waits for the graph
+    for (size_t g = 1; g < ng; g++)
+        if (hipStreamWaitEvent(e->batch[g]->st, done, 0) != hipSuccess)
+            return efail(e, HGPM_ERR_HIP, "hgpm_ensemble_run: stream ordering failed");
     return HGPM_OK;
 }
 
@@ -1499,6 +1609,7 @@ int hgpm_ensemble_synchronize(hgpm_ensemble* e) {
         int rc = hgpm_synchronize(c);
         if (rc) return eforward(e, c, rc);
     }
+    release_graphs(e, true);
     return HGPM_OK;
 }
 

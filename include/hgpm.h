@@ -287,6 +287,11 @@ int hgpm_ensemble_get_particles(hgpm_ensemble* e, int box, float* pos, float* ve
 int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k, double* pk, int64_t* count);
 /* One step of every box (hgpm_step); coeffs[b] for box b, equal within a batch. */
 int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs);
+/* nsteps steps of every box captured into one HIP graph and launched once;
+   coeffs[s * n_boxes + b] for step s, box b.  Bit-identical to nsteps calls of
+   hgpm_ensemble_step; asynchronous like it (hgpm_ensemble_synchronize waits).  Per-kernel
+   timing (hgpm_set_timing) does not see the launches inside the graph. */
+int hgpm_ensemble_run(hgpm_ensemble* e, int nsteps, const hgpm_step_coeffs* coeffs);
 int hgpm_ensemble_synchronize(hgpm_ensemble* e);
 /* hgpm_autotune on every batch. */
 int hgpm_ensemble_autotune(hgpm_ensemble* e, int reps);

@@ -537,6 +537,52 @@ def test_ensemble_matches_single_boxes(hg):
 
 
 
+def test_ensemble_graph_run_matches_steps(hg):
+    """Steps captured into one HIP graph per run (hgpm_ensemble_run) against the same
+    steps launched one by one: bit-identical particles and P(k) after each of three
+    runs, with eager per-box work (P(k), readback) between the runs.  Two batches on
+    two streams, and a third ensemble stored in waves."""
+    case("ensemble graph run: 2 x 32^3 + 64^3, and 4 x 64^3 in waves")
+    ps = [hg.make_params(32, 32, 64.0, 30.0, 0.0, 40, seed=31, ss8=0.80),
+          hg.make_params(32, 32, 64.0, 30.0, 0.0, 40, seed=32, ss8=0.72),
+          hg.make_params(64, 64, 128.0, 30.0, 0.0, 40, seed=33, ss8=0.88)]
+    qs = [hg.make_params(64, 64, 128.0, 30.0, 0.0, 40, seed=40 + b, ss8=0.7 + 0.05 * b) for b in range(4)]
+    n_steps, seg = 6, 2
+    with hg.Ensemble(ps) as eager, hg.Ensemble(ps) as graph, \
+            hg.Ensemble(qs) as q_eager, hg.Ensemble(qs, device_budget=120 << 20) as q_graph:
+        assert eager.batches()[0] == 2
+        assert int(q_graph.waves()[0]) == 2
+        pairs = [(eager, graph, ps), (q_eager, q_graph, qs)]
+        scheds = {}
+        for a, g, pl in pairs:
+            sc = []
+            for b, p in enumerate(pl):
+                hdr, vals, delta, dot_delta, ts = _ic_inputs(hg, p)
+                a.generate_ic(b, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+                g.generate_ic(b, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+                sc.append(hg.schedule(p, n_steps, merge=True))
+            scheds[id(a)] = sc
+        for s0 in range(0, n_steps, seg):
+            for a, g, pl in pairs:
+                sc = scheds[id(a)]
+                for s in range(s0, s0 + seg):
+                    a.step([sc[b][s] for b in range(len(pl))])
+                g.run([[sc[b][s] for b in range(len(pl))] for s in range(s0, s0 + seg)])
+            for a, g, pl in pairs:
+                for b in range(len(pl)):
+                    p1, v1 = a.get_particles(b)
+                    p2, v2 = g.get_particles(b)
+                    assert np.array_equal(p1, p2) and np.array_equal(v1, v2), (s0, b)
+                    assert np.array_equal(a.power_spectrum(b)[1], g.power_spectrum(b)[1]), (s0, b)
+        graph.synchronize()
+        bad = [scheds[id(eager)][b][0] for b in range(len(ps))]
+        c = hg.StepCoeffs()
+        c.drift0, c.kick_dt, c.kick_fscal, c.drift1, c.skip_drift0 = 0.1, 0.2, 0.3, 0.4, 0
+        bad[1] = c
+        with pytest.raises(hg.HgpmError):
+            graph.run([bad])
+
+
 def test_ensemble_waves_match_resident(hg):
     """An ensemble squeezed into a small device budget keeps every box's particles but
     grid buffers for a wave of boxes only, and steps the waves in turn: bit-identical
