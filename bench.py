@@ -112,6 +112,53 @@ def run_engine(hg, p, steps, warmup, timing=True, gather_tile=-1):
     return el, tim, t_ic, dev_bytes, tune
 
 
+def run_full(hg, p, per_kernel=True):
+    """The whole run of the workload, z_ini -> z_fin in p.nsteps steps from fresh initial
+    conditions, each step timed on its own (host clock, the stream synchronised after
+    every step): clustering grows toward z = 0, and with it the deposit's windows.
+    per_kernel: the kernel regions' HIP-event times are averaged over the first and the
+    last 10% of the steps as well."""
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    hdr, vals = make_ics(hg, p, ts)
+    n = p.nsteps
+    sched = hg.schedule(p, n, merge=True)
+    t = []
+    deferred = []
+    regions = []
+    with hg.MemoryManager(p) as m:
+        m.autotune(3)
+        hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
+        m.synchronize()
+        for s in range(n):
+            m.set_timing(per_kernel)
+            m.reset_timings()
+            t0 = time.perf_counter()
+            m.step(sched[s])
+            m.synchronize()
+            t.append(time.perf_counter() - t0)
+            regions.append(m.timings() if per_kernel else {})
+            deferred.append([m.get_tunable(k) for k in
+                             ("deposit_deferred", "deposit_wide", "deposit_deferred_full")])
+    k = max(1, n // 10)
+    Np = p.np ** 3
+
+    def region_ms(rs):
+        names = sorted({name for r in rs for name in r})
+        return {name: round(sum(r.get(name, (0.0, 0))[0] for r in rs) / len(rs), 3) for name in names}
+
+    return {"steps": n, "z": [p.z_ini, p.z_fin], "ms_per_step_mean": 1e3 * sum(t) / n,
+            "ms_per_step_first_10pct": 1e3 * sum(t[:k]) / k, "ms_per_step_last_10pct": 1e3 * sum(t[-k:]) / k,
+            "particle_steps_per_s": Np * n / sum(t), "deferred_tiles_max": max(d[0] for d in deferred),
+            # per step: tiles deferred, tiles with 64-bit windows, tiles deferred for a full
+            # slot list or sub-arena
+            "deferred_tiles": deferred,
+            "per_kernel": per_kernel,
+            "kernel_ms_first_10pct": region_ms(regions[:k]), "kernel_ms_last_10pct": region_ms(regions[-k:]),
+            "ms_per_step": [round(1e3 * x, 3) for x in t]}
+
+
 def cpu_baseline(hg, threads, ng=192):
     """Reference algorithm on the host: the oracle in the reference's FP32 rounding
     (float3 positions, FP32 CIC, reference calcGreens), on a bounded sample of the same
@@ -247,13 +294,18 @@ def run_ensemble(hg, steps, warmup, per_class):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    # the single-box run is 50 steps from z = 50 to z = 0 and the ensemble's 40 steps from
+    # z = 30 to z = 0: by default the 3 first are the warm-up and the others (through
+    # z = 0) are timed
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ng", type=int, default=1024)
     ap.add_argument("--rl", type=float, default=2000.0)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true",
+                    help="skip the whole z_ini -> z_fin run timed step by step")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device twice (a plain copy kernel of known "
@@ -281,11 +333,15 @@ def main():
     import hacc_gpm_amd as hg
 
     if args.ensemble:
-        print(json.dumps(run_ensemble(hg, args.steps, args.warmup, args.per_class)))
+        steps = args.steps if args.steps is not None else max(1, 40 - args.warmup)
+        print(json.dumps(run_ensemble(hg, steps, args.warmup, args.per_class)))
         return
+    if args.steps is None:
+        args.steps = max(1, 50 - args.warmup)
     ng = args.ng
     workload = f"pm_step_{ng}^3_particles_{ng}^3_grid_box{int(args.rl)}"
-    p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221)
+    n_run = max(50, args.warmup + args.steps) # steps of the whole run, z = 50 -> 0
+    p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221)
     Np = ng ** 3
     G = ng ** 3
     el, tim, t_ic, dev_bytes, tune = run_engine(hg, p, args.steps, args.warmup, gather_tile=args.gather_tile)
@@ -324,7 +380,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: Zel'dovich ICs at z=50 from Philox white noise and an Eisenstein-Hu no-wiggle P(k)",
         "config": {"workload": workload, "np": ng, "ng": ng,
-                   "rl": args.rl, "z_ini": 50, "steps_in_run": 50, "positions": "int32 cell + f32 offset",
+                   "rl": args.rl, "z_ini": 50, "z_fin": 0, "steps_in_run": n_run,
+                   "timed_steps": [args.warmup, args.warmup + args.steps - 1], "positions": "int32 cell + f32 offset",
                    "deposit": "int64 fixed point 2^-24"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -342,8 +399,10 @@ def main():
         "ic_seconds": t_ic,
         "device_GB": dev_bytes / 1e9,
     }
+    if not args.no_full_run:
+        out["full_run"] = run_full(hg, p)
     if not args.no_fp64:
-        p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, 50, seed=2, pk_bins=221, precision=hg.FP64)
+        p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221, precision=hg.FP64)
         try:
             el64, _, _, b64, _ = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
             v64 = Np * args.fp64_steps / el64

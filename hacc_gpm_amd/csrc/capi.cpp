@@ -60,6 +60,8 @@ struct hgpm_ctx {
     void* dslots = nullptr;
     unsigned* dcount = nullptr;
     unsigned* deferred = nullptr;
+    unsigned* wide = nullptr;
+    unsigned short* wide_slots = nullptr;
     unsigned long long* tile_total = nullptr;
     void* boxes = nullptr;   // half-tile cell boxes of the last windows pass
     bool boxes_valid = false; // ... and the positions have not changed since
@@ -232,6 +234,8 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.count = c->dcount;
     b.nslots = c->nslots;
     b.deferred = c->deferred;
+    b.wide = c->wide;
+    b.wide_slots = c->wide_slots;
     b.ctl = c->ctl;
     b.tile_total = c->tile_total;
     b.boxes = c->boxes;
@@ -466,6 +470,9 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
                 (rc = dalloc(c, &c->dcount, sizeof(unsigned) * nblk, "deposit counts")) ||
                 (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||
+                (rc = dalloc(c, &c->wide, sizeof(unsigned) * ntiles, "deposit 64-bit list")) ||
+                (rc = dalloc(c, &c->wide_slots, sizeof(unsigned short) * HGPM_DEP_WKEEP * ntiles,
+                             "deposit 64-bit slots")) ||
                 (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
                 (rc = dalloc(c, (char**)&c->boxes, 64 * ntiles, "gather boxes")))
                 return bail(rc);
@@ -533,7 +540,7 @@ int hgpm_create(const hgpm_params* p, hgpm_ctx** out) { return ctx_create(p, 1, 
 int hgpm_destroy(hgpm_ctx* c) {
     if (!c) return HGPM_OK;
     if (c->st) (void)hipStreamSynchronize(c->st);
-    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->psi, c->ctl, c->totals, c->arena, c->dslots, c->dcount, c->deferred, c->tile_total, c->boxes, c->twf, c->twr, c->hs,
+    void* ptrs[] = {c->pos, c->vel, c->A, c->B, c->psi, c->ctl, c->totals, c->arena, c->dslots, c->dcount, c->deferred, c->wide, c->wide_slots, c->tile_total, c->boxes, c->twf, c->twr, c->hs,
                     c->pos64, c->vel64, c->grid64, c->grad64, c->tw64, c->pk_partial, c->pk_sums,
                     c->pk_counts, c->pk_win, c->flag};
     for (void* q : ptrs)
@@ -1184,14 +1191,18 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
     if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
-    if (!strcmp(key, "deposit_deferred")) { // tiles that took the atomic route in the last deposit
-        if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
-        unsigned long long v[2];
-        HIPCHK(c, hipMemcpyAsync(v, c->ctl, sizeof(v), hipMemcpyDeviceToHost, c->st));
-        HIPCHK(c, hipStreamSynchronize(c->st));
-        *value = (int)v[1];
-        return HGPM_OK;
-    }
+    // last deposit: tiles that took the atomic route, tiles whose windows were stored
+    // with 64-bit cells, deferred tiles that found their slot list or sub-arena full
+    const char* dkeys[3] = {"deposit_deferred", "deposit_wide", "deposit_deferred_full"};
+    for (int i = 0; i < 3; i++)
+        if (!strcmp(key, dkeys[i])) {
+            if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
+            unsigned long long v[4];
+            HIPCHK(c, hipMemcpyAsync(v, c->ctl, sizeof(v), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            *value = (int)v[1 + i];
+            return HGPM_OK;
+        }
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
 }
 
@@ -1324,7 +1335,7 @@ size_t batch_bytes(const hgpm_params& p, int nstore, int nwave) {
     b += (2 * sizeof(float2) + sizeof(float)) * G * nwave;              // A, B, psi
     b += 6 * sizeof(unsigned) * G * nwave;                              // arena
     b += (16 * HGPM_DEP_SLOTS + sizeof(unsigned)) * nblk;               // slots, counts
-    b += (sizeof(unsigned) + sizeof(unsigned long long) + 64) * ntiles; // tile buffers
+    b += (2 * sizeof(unsigned) + sizeof(unsigned long long) + 64 + 2 * HGPM_DEP_WKEEP) * ntiles; // tile buffers
     b += 64ull << 20;                                                   // tables, P(k), slack
     return b;
 }

@@ -59,14 +59,17 @@ void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf, hipStream_t s);
 // Deposit buffers.  np3 is a multiple of HGPM_TL^3 and ng of 16 (supported sizes).
 //   arena    : cap 32-bit cells for the tiles' windows (a tile with a cell over 2^32 - 1
-//              mass units takes the atomic route);
+//              mass units stores its window again with 64-bit cells);
 //   slots    : HGPM_DEP_SLOTS chunk descriptors (16 B) per 16^3 owner block;
 //   count    : chunks listed per owner block, zero between deposits;
 //   deferred : one entry per tile;
-//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] deposited total,
-//              [1] deferred tiles, [4 + s] cells reserved in sub-arena s;
+//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [1] deferred tiles, of
+//              them [3] for a full slot list or sub-arena (the others had too many
+//              chunks); [2] tiles stored with 64-bit cells; [4 + s] cells reserved in
+//              sub-arena s;
 //   tile_total: deposited mass per tile.
 #define HGPM_DEP_SLOTS 64
+#define HGPM_DEP_WKEEP 32
 #define HGPM_DEP_NSUB 64
 #define HGPM_DEP_CTL (4 + HGPM_DEP_NSUB)
 struct DepositBuffers {
@@ -76,6 +79,8 @@ struct DepositBuffers {
     unsigned* count;
     int nslots;
     unsigned* deferred;
+    unsigned* wide; // tiles stored again with 64-bit cells: one entry per tile
+    unsigned short* wide_slots; // HGPM_DEP_WKEEP slot indices per entry of wide
     unsigned long long* ctl;
     unsigned long long* tile_total;
     int nsub; // sub-arenas in use, 1..HGPM_DEP_NSUB

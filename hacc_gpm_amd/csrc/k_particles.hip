@@ -97,7 +97,9 @@ template <bool SMALL> __device__ __forceinline__ DepW<SMALL> fixed_weight_w(floa
 // The finished window cells reach the density grid without global atomics:
 //   k_deposit_windows : stores each tile's window to an arena with plain stores and cuts
 //                       it along the OB^3 owner blocks of the grid; each piece (chunk)
-//                       gets a descriptor in its owner block's slot list;
+//                       gets a descriptor in its owner block's slot list.  Cells are
+//                       32-bit; a tile with a cell over 2^32 - 1 mass units stores its
+//                       window again with 64-bit cells and flags its descriptors;
 //   k_deposit_owner   : one workgroup per owner block adds the chunks listed for it in
 //                       LDS and writes the block's cells once (the grid is not cleared
 //                       beforehand: every cell is written);
@@ -107,9 +109,10 @@ template <bool SMALL> __device__ __forceinline__ DepW<SMALL> fixed_weight_w(floa
 constexpr int DEP_NT = 512;
 constexpr int DEP_PPT = 8;
 constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
-constexpr int DEP_LDS = 65536;             // window bytes per workgroup (two per CU)
+constexpr int DEP_LDS = 79872;             // window bytes per workgroup (two per CU: 160 KB)
 constexpr int OB = 16;                     // owner block edge (cells)
 constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
+constexpr int DEP_WKEEP = HGPM_DEP_WKEEP;  // slots kept per tile for the 64-bit pass
 static_assert(DEP_MAXCH <= DEP_NT, "one chunk per thread");
 constexpr int OWN_NT = 256;
 constexpr int OWN_U = 16;               // arena loads in flight per lane
@@ -249,7 +252,7 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
 // cell is stored to arena[base + window index] (window order x, y, z with extents e)
 // as a 32-bit value.  dep and fl receive this thread's sums of the deposited weights
 // and of the stored cells: the workgroup totals differ exactly when a cell exceeded
-// 2^32 - 1 (the caller then sends the tile to the atomic route).
+// 2^32 - 1 (the caller then stores the window again with 64-bit cells).
 // A window of at most DEP_LDS bytes is accumulated in one pass (dep_one_accumulate,
 // then dep_one_flush: a contiguous arena range); a larger one slab by slab
 // (dep_window_arena_slabs).
@@ -271,11 +274,20 @@ __device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRe
 }
 // after a barrier that follows dep_one_accumulate
 // base is a multiple of 4 cells (reservations are): whole 4-cell groups move as 16 bytes.
-template <bool SMALL>
+// WIDE (64-bit window): cell i is stored whole at arena words base + 2i, base + 2i + 1.
+template <bool SMALL, bool WIDE = false>
 __device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int e[3], unsigned* __restrict arena,
                                               unsigned long long base, unsigned long long& fl) {
+    static_assert(!WIDE || !SMALL, "64-bit cells need the 64-bit window");
     const int vol = e[0] * e[1] * e[2];
-    if constexpr (SMALL) {
+    if constexpr (WIDE) {
+        unsigned long long* __restrict out = (unsigned long long*)(arena + base);
+        for (int i = threadIdx.x; i < vol; i += DEP_NT) {
+            const unsigned long long v = win[i];
+            fl += v;
+            out[i] = v;
+        }
+    } else if constexpr (SMALL) {
         unsigned* __restrict out = arena + base;
         for (int i = 4 * threadIdx.x; i < vol; i += 4 * DEP_NT) {
             const uint4 v = *(const uint4*)&win[i];
@@ -298,13 +310,17 @@ __device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int 
         }
     }
 }
-template <bool SMALL>
+// WIDE (64-bit LDS window, !SMALL): every cell is stored whole as two 32-bit arena
+// words, cell i of the window at words base + 2i, base + 2i + 1.
+template <bool SMALL, bool WIDE = false>
 __device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
                                                        const int e[3], float mass, unsigned* __restrict arena,
                                                        unsigned long long base, unsigned long long& dep,
                                                        unsigned long long& fl) {
     typedef DepW<SMALL> W;
+    static_assert(!WIDE || !SMALL, "64-bit cells need the 64-bit window");
     constexpr int WM = DEP_LDS / (int)sizeof(W);
+    unsigned long long* __restrict wide = (unsigned long long*)(arena + base); // base: 4-cell aligned
     const int tid = threadIdx.x;
     dep_slabs<WM>(e, [&](int sx0, int nx, int sy0, int ny) {
         const int ez = e[2], svol = nx * ny * ez;
@@ -318,13 +334,18 @@ __device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const D
         const int sz = DEP_NT % ez, syx = DEP_NT / ez, sy = syx % ny, sx = syx / ny;
         int iz = tid % ez, iy = (tid / ez) % ny, ix = (tid / ez) / ny;
         const int64_t plane = (int64_t)e[1] * ez;
-        int64_t ai = (int64_t)base + ((int64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz;
+        int64_t ai = ((int64_t)(sx0 + ix) * e[1] + (sy0 + iy)) * ez + iz; // window index
         const int64_t astep = sz + (int64_t)sy * ez + (int64_t)sx * plane;
         const int64_t askip = plane - (int64_t)ny * ez;
         for (int i = tid; i < svol; i += DEP_NT) {
             const W v = win[i];
-            fl += (unsigned)v;
-            arena[ai] = (unsigned)v;
+            if constexpr (WIDE) {
+                fl += v;
+                wide[ai] = v;
+            } else {
+                fl += (unsigned)v;
+                arena[base + ai] = (unsigned)v;
+            }
             iz += sz;
             if (iz >= ez) { iz -= ez; iy += 1; }
             iy += sy;
@@ -410,36 +431,66 @@ __device__ __forceinline__ Chunk dep_chunk(int c, const int A[3], const int e[3]
     return ch;
 }
 
-// ctl[1] = deferred tiles, ctl[DEP_CTL_SUB + s] = cells reserved in sub-arena s.  The
+// ctl[1] = deferred tiles, ctl[2] = tiles listed in wide, ctl[3] = tiles deferred for a
+// full slot list or sub-arena, ctl[DEP_CTL_SUB + s] = cells reserved in sub-arena s.  The
 // arena is split into nsub <= DEP_NSUB equal sub-arenas (tile % nsub picks one) so that
 // no single counter takes every tile's reservation; a few hundred tiles per sub-arena
 // let their window sizes average out.
-template <bool SMALL>
-__global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __restrict pos, int64_t n, int ng,
-                                                            float mass, unsigned* __restrict arena,
-                                                            unsigned long long cap, uint4* __restrict slots,
-                                                            unsigned* __restrict count, int nslots,
-                                                            unsigned* __restrict deferred,
-                                                            unsigned long long* __restrict ctl,
-                                                            unsigned long long* __restrict tile_total, int nsub,
-                                                            int4* __restrict boxes, int ntb, int nblk) {
-    __shared__ __align__(16) unsigned long long win_raw[DEP_LDS / 8];
-    DepW<SMALL>* win = (DepW<SMALL>*)win_raw;
-    __shared__ int red[12][DEP_NT / 64];
-    __shared__ unsigned long long s_tot[DEP_NT / 64], s_dif[DEP_NT / 64];
-    __shared__ unsigned long long s_base;
-    __shared__ int s_over;  // a window cell did not fit the 32-bit arena
-    __shared__ int s_full;  // a slot list or the sub-arena was full (read after the publishing barrier)
-    __shared__ unsigned short s_slot[DEP_MAXCH];
+struct DepWindowArgs {
+    const PosPack* __restrict pos;
+    int ng;
+    float mass;
+    unsigned* __restrict arena;
+    unsigned long long cap;
+    uint4* __restrict slots;
+    unsigned* __restrict count;
+    int nslots;
+    unsigned* __restrict deferred;
+    unsigned* __restrict wide; // tiles for the 64-bit pass
+    // per entry of wide: the slots of the tile's first DEP_WKEEP chunks (voided by the
+    // 32-bit pass), taken again by the 64-bit pass
+    unsigned short* __restrict wide_slots;
+    unsigned long long* __restrict ctl;
+    unsigned long long* __restrict tile_total;
+    int nsub;
+    int4* __restrict boxes;
+    int ntb;  // tiles per box
+    int nblk; // owner blocks per box
+};
+// Workgroup-shared state of the windows pass.
+struct DepWindowShared {
+    unsigned long long win[DEP_LDS / 8];
+    int red[12][DEP_NT / 64];
+    unsigned long long tot[DEP_NT / 64], dif[DEP_NT / 64];
+    unsigned long long base;
+    int over; // a window cell did not fit the 32-bit arena
+    int full; // a slot list or the sub-arena was full (read after the publishing barrier)
+    unsigned q; // entry of wide
+    unsigned short slot[DEP_MAXCH];
+};
+
+// The windows pass of one tile.  WIDE: the 32-bit pass found a window cell over
+// 2^32 - 1 mass units, voided the tile's chunks and listed it in wide; the window is
+// accumulated again with 64-bit cells, stored whole (two arena words per cell, the
+// reservation doubled) and its chunks listed with the wide flag (descriptor bit 30).
+// Only the 32-bit pass stores the gather's boxes.  q: the tile's entry of wide (WIDE);
+// a tile of at most DEP_WKEEP chunks takes the slots it had in the 32-bit pass again
+// (the same chunks, in the same order), a larger one lists its chunks anew.
+template <bool SMALL, bool WIDE>
+__device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const DepWindowArgs& g,
+                                                 DepWindowShared& sh) {
+    constexpr bool S32 = SMALL && !WIDE;           // 32-bit LDS window
+    constexpr unsigned long long CW = WIDE ? 2 : 1; // arena words per cell
+    DepW<S32>* win = (DepW<S32>*)sh.win;
     const int tid = threadIdx.x;
-    const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
+    const int ng = g.ng;
     const int64_t p0 = (int64_t)tile * DEP_TILE;
-    const PosSplit r0 = unpack(pos[p0]);
-    if (tid == 0) { s_over = 0; s_full = 0; } // ordered by the barrier in dep_load_box
+    const PosSplit r0 = unpack(g.pos[p0]);
+    if (tid == 0) { sh.over = 0; sh.full = 0; } // ordered by the barrier in dep_load_box
     DepRegs R;
     int lo[3], e[3], hb[2][6];
-    dep_load_box(pos, p0, n, ng, r0, R, red, lo, e, hb);
-    if (tid == 0) {
+    dep_load_box(g.pos, p0, 0, ng, r0, R, sh.red, lo, e, hb);
+    if (!WIDE && tid == 0) {
         // the gather of these positions stages its potential window from these boxes.
         // Their origins are stored as absolute cells (centred into [-ng/2, ng/2)): the
         // gather must not re-read r0, whose particle another gather workgroup drifts.
@@ -452,10 +503,10 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
                 int a = wrap_cell(rc + hb[h][d], ng);
                 o[h][d] = a >= ng / 2 ? a - ng : a;
             }
-        boxes[4 * tile + 0] = make_int4(o[0][0], o[0][1], o[0][2], hb[0][3]);
-        boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
-        boxes[4 * tile + 2] = make_int4(o[1][0], o[1][1], o[1][2], hb[1][3]);
-        boxes[4 * tile + 3] = make_int4(hb[1][4], hb[1][5], 0, 0);
+        g.boxes[4 * tile + 0] = make_int4(o[0][0], o[0][1], o[0][2], hb[0][3]);
+        g.boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
+        g.boxes[4 * tile + 2] = make_int4(o[1][0], o[1][1], o[1][2], hb[1][3]);
+        g.boxes[4 * tile + 3] = make_int4(hb[1][4], hb[1][5], 0, 0);
     }
     const int nb = ng / OB;
     const int A[3] = {r0.cx + lo[0], r0.cy + lo[1], r0.cz + lo[2]};
@@ -467,89 +518,113 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(const PosPack* __res
     }
     const int nch = nc[0] * nc[1] * nc[2];
     // arena reservation, a multiple of 4 cells so that every window starts 16-byte aligned
-    const unsigned long long V = ((unsigned long long)e[0] * e[1] * e[2] + 3ull) & ~3ull;
+    const unsigned long long V = CW * (((unsigned long long)e[0] * e[1] * e[2] + 3ull) & ~3ull);
     // owner blocks of this tile's box (a batch holds several boxes of the same shape)
-    const int blk0 = (tile / ntb) * nblk;
+    const int blk0 = (tile / g.ntb) * g.nblk;
     // The sub-arena reservation and the chunk listing are issued first and their
     // results published after a one-pass window is accumulated, so that their memory
     // latency overlaps the accumulation.  A tile with more than DEP_MAXCH chunks takes
     // the atomic route and lists nothing.
     const bool fits = nch <= DEP_MAXCH;
     unsigned long long rb = 0;
-    if (fits && tid == 0) rb = atomicAdd(&ctl[DEP_CTL_SUB + tile % nsub], V);
+    if (fits && tid == 0) rb = atomicAdd(&g.ctl[DEP_CTL_SUB + tile % g.nsub], V);
     unsigned sl = 0xFFFFu; // DEP_MAXCH <= DEP_NT: at most one chunk per thread
     if (fits && tid < nch) {
-        const Chunk ch = dep_chunk(tid, A, e, c0, nc, nb);
-        sl = atomicAdd(&count[blk0 + ch.blk], 1u);
+        if (WIDE && nch <= DEP_WKEEP) {
+            sl = g.wide_slots[(size_t)q * DEP_WKEEP + tid];
+        } else {
+            const Chunk ch = dep_chunk(tid, A, e, c0, nc, nb);
+            sl = atomicAdd(&g.count[blk0 + ch.blk], 1u);
+        }
     }
     unsigned long long dep = 0, fl = 0;
-    const bool one = dep_one_pass<SMALL>(e);
-    if (one) dep_one_accumulate<SMALL>(win, R, lo, e, mass, dep);
+    const bool one = dep_one_pass<S32>(e);
+    if (one) dep_one_accumulate<S32>(win, R, lo, e, g.mass, dep);
     if (fits) {
         if (tid == 0) {
-            const unsigned long long sub_cap = (cap / nsub) & ~3ull;
-            s_base = (unsigned long long)(tile % nsub) * sub_cap + rb;
-            if (rb + V > sub_cap) s_full = 1;
+            const unsigned long long sub_cap = (g.cap / g.nsub) & ~3ull;
+            sh.base = (unsigned long long)(tile % g.nsub) * sub_cap + rb;
+            if (rb + V > sub_cap) sh.full = 1;
         }
         if (tid < nch) {
-            s_slot[tid] = (unsigned short)(sl < (unsigned)nslots ? sl : 0xFFFFu);
-            if (sl >= (unsigned)nslots) s_full = 1;
+            sh.slot[tid] = (unsigned short)(sl < (unsigned)g.nslots ? sl : 0xFFFFu);
+            if (sl >= (unsigned)g.nslots) sh.full = 1;
         }
     }
     __syncthreads();
     // 0: arena route; 1: deferred, nothing listed; 2: deferred, listed slots voided
-    const int state = !fits ? 1 : (s_full ? 2 : 0);
+    const int state = !fits ? 1 : (sh.full ? 2 : 0);
     if (state != 1) {
-        const unsigned long long base = s_base;
+        const unsigned long long base = sh.base;
         for (int c = tid; c < nch; c += DEP_NT) {
-            if (s_slot[c] == 0xFFFFu) continue;
+            if (sh.slot[c] == 0xFFFFu) continue;
             const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
-            const unsigned long long off = base + ((uint64_t)ch.w[0] * e[1] + ch.w[1]) * e[2] + ch.w[2];
+            const unsigned long long off = base + CW * (((uint64_t)ch.w[0] * e[1] + ch.w[1]) * e[2] + ch.w[2]);
             uint4 de;
             de.x = (unsigned)off;
             de.y = (unsigned)(off >> 32);
             de.z = (unsigned)(ch.o[0] | ch.o[1] << 4 | ch.o[2] << 8 | (ch.m[0] - 1) << 12 | (ch.m[1] - 1) << 16 |
                               (ch.m[2] - 1) << 20) |
-                   (state == 2 ? 0x80000000u : 0u);
+                   (state == 2 ? 0x80000000u : 0u) | (WIDE ? 0x40000000u : 0u);
             de.w = (unsigned)e[1] | (unsigned)e[2] << 16;
-            slots[(size_t)(blk0 + ch.blk) * nslots + s_slot[c]] = de;
+            g.slots[(size_t)(blk0 + ch.blk) * g.nslots + sh.slot[c]] = de;
         }
     }
     if (state != 0) {
         if (tid == 0) {
-            deferred[atomicAdd(&ctl[1], 1ull)] = (unsigned)tile;
-            tile_total[tile] = 0ull; // written by k_deposit_deferred
+            g.deferred[atomicAdd(&g.ctl[1], 1ull)] = (unsigned)tile;
+            if (state == 2) atomicAdd(&g.ctl[3], 1ull);
+            g.tile_total[tile] = 0ull; // written by k_deposit_deferred
         }
         return;
     }
-    if (one) dep_one_flush<SMALL>(win, e, arena, s_base, fl);
-    else dep_window_arena_slabs<SMALL>(win, R, lo, e, mass, arena, s_base, dep, fl);
+    if (one) dep_one_flush<S32, WIDE>(win, e, g.arena, sh.base, fl);
+    else dep_window_arena_slabs<S32, WIDE>(win, R, lo, e, g.mass, g.arena, sh.base, dep, fl);
     // workgroup totals: the stored cells sum to the deposited weights unless a cell wrapped
     unsigned long long dif = dep - fl;
     for (int off = 32; off > 0; off >>= 1) {
         dep += __shfl_down(dep, off, 64);
         dif += __shfl_down(dif, off, 64);
     }
-    if ((tid & 63) == 0) { s_tot[tid >> 6] = dep; s_dif[tid >> 6] = dif; }
+    if ((tid & 63) == 0) { sh.tot[tid >> 6] = dep; sh.dif[tid >> 6] = dif; }
     __syncthreads();
     if (tid == 0) {
         unsigned long long t = 0, d = 0;
-        for (int w = 0; w < DEP_NT / 64; w++) { t += s_tot[w]; d += s_dif[w]; }
-        s_over = d != 0ull;
-        if (!d) tile_total[tile] = t;
+        for (int w = 0; w < DEP_NT / 64; w++) { t += sh.tot[w]; d += sh.dif[w]; }
+        sh.over = d != 0ull;
+        if (!d) g.tile_total[tile] = t;
     }
     __syncthreads();
-    if (s_over) {
-        // a cell over 2^32 - 1 mass units: void this tile's chunks, atomic route instead
-        for (int c = tid; c < nch; c += DEP_NT)
-            if (s_slot[c] != 0xFFFFu) {
-                const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
-                slots[(size_t)(blk0 + ch.blk) * nslots + s_slot[c]] = make_uint4(0u, 0u, 0x80000000u, 0u);
-            }
+    if (!WIDE && sh.over) {
+        // a cell over 2^32 - 1 mass units: void this tile's chunks (keeping their slots
+        // for the 64-bit pass), 64-bit pass instead
         if (tid == 0) {
-            deferred[atomicAdd(&ctl[1], 1ull)] = (unsigned)tile;
-            tile_total[tile] = 0ull; // written by k_deposit_deferred
+            sh.q = (unsigned)atomicAdd(&g.ctl[2], 1ull);
+            g.wide[sh.q] = (unsigned)tile;
         }
+        __syncthreads();
+        for (int c = tid; c < nch; c += DEP_NT) {
+            const Chunk ch = dep_chunk(c, A, e, c0, nc, nb);
+            g.slots[(size_t)(blk0 + ch.blk) * g.nslots + sh.slot[c]] = make_uint4(0u, 0u, 0x80000000u, 0u);
+            if (c < DEP_WKEEP) g.wide_slots[(size_t)sh.q * DEP_WKEEP + c] = sh.slot[c];
+        }
+    }
+}
+
+template <bool SMALL>
+__global__ __launch_bounds__(DEP_NT) void k_deposit_windows(DepWindowArgs g) {
+    __shared__ __align__(16) DepWindowShared sh;
+    dep_windows_tile<SMALL, false>((int)xcd_remap(blockIdx.x, gridDim.x), 0u, g, sh);
+}
+
+// The tiles the 32-bit pass listed in wide (ctl[2] of them), before the owner pass.
+template <bool SMALL>
+__global__ __launch_bounds__(DEP_NT) void k_deposit_wide(DepWindowArgs g) {
+    __shared__ __align__(16) DepWindowShared sh;
+    const unsigned long long nw = g.ctl[2];
+    for (unsigned long long q = blockIdx.x; q < nw; q += gridDim.x) {
+        dep_windows_tile<SMALL, true>((int)g.wide[q], (unsigned)q, g, sh);
+        __syncthreads();
     }
 }
 
@@ -617,19 +692,33 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
             const int ly = (int)(((float)pl + 0.5f) * rmz), lz = pl - ly * mz;
             const unsigned po = pv ? (unsigned)(ly * e2 + lz) : 0u; // arena offset in the plane
             const int la = ox * OB * OB + (oy + ly) * OB + oz + lz; // LDS cell of x plane 0
+            if (de.z & 0x40000000u) {
+                // a window of 64-bit cells (two arena words per cell)
+                for (int x0 = 0; x0 < mx; x0 += OWN_U) {
+                    unsigned long long v[OWN_U];
+#pragma unroll
+                    for (int u = 0; u < OWN_U; u++)
+                        if (x0 + u < mx)
+                            v[u] = *(const unsigned long long*)((const char*)ch +
+                                                               ((unsigned)(x0 + u) * e12 + po) * 8u);
+#pragma unroll
+                    for (int u = 0; u < OWN_U; u++)
+                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], v[u]);
+                }
+                continue;
+            }
             for (int x0 = 0; x0 < mx; x0 += OWN_U) {
                 unsigned v[OWN_U];
 #pragma unroll
                 for (int u = 0; u < OWN_U; u++) {
                     // uniform bound; lanes past the plane load its first cell.  A chunk
                     // lies in a window of at most DEP_MAXCH owner blocks: byte offsets
-                    // from its first cell fit 32 bits
+                    // from its first cell fit 32 bits (also for 64-bit cells)
                     if (x0 + u < mx)
                         v[u] = *(const unsigned*)((const char*)ch + ((unsigned)(x0 + u) * e12 + po) * 4u);
                 }
 #pragma unroll
                 for (int u = 0; u < OWN_U; u++) {
-                    // 32-bit arena cells: tiles whose cells exceed them took the atomic route
                     if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (unsigned long long)v[u]);
                 }
             }
@@ -1046,20 +1135,38 @@ void launch_drift(PosPackH* pos, const float* vel, int64_t n, int ng, float pf, 
     if (!n) return;
     k_drift<<<grid_blocks(n, BS), BS, 0, s>>>((PosPack*)pos, vel, n, ng, pf);
 }
+DepWindowArgs window_args(const PosPackH* pos, int64_t n, int ng, float mass, const DepositBuffers& b) {
+    DepWindowArgs g;
+    g.pos = (const PosPack*)pos;
+    g.ng = ng;
+    g.mass = mass;
+    g.arena = b.arena;
+    g.cap = b.cap;
+    g.slots = (uint4*)b.slots;
+    g.count = b.count;
+    g.nslots = b.nslots;
+    g.deferred = b.deferred;
+    g.wide = b.wide;
+    g.wide_slots = b.wide_slots;
+    g.ctl = b.ctl;
+    g.tile_total = b.tile_total;
+    g.nsub = b.nsub;
+    g.boxes = (int4*)b.boxes;
+    g.ntb = (int)(n / b.nbox / DEP_TILE);
+    g.nblk = (ng / OB) * (ng / OB) * (ng / OB);
+    return g;
+}
 void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s) {
     if (!n) return;
     const unsigned nt = (unsigned)(n / DEP_TILE);
-    const int ntb = (int)(n / b.nbox / DEP_TILE);
-    const int nblk = (ng / OB) * (ng / OB) * (ng / OB);
-    if (mass < 128.0f)
-        k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, b.arena, b.cap,
-                                                      (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                      b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
-    else
-        k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, b.arena, b.cap,
-                                                       (uint4*)b.slots, b.count, b.nslots, b.deferred, b.ctl,
-                                                       b.tile_total, b.nsub, (int4*)b.boxes, ntb, nblk);
+    const DepWindowArgs g = window_args(pos, n, ng, mass, b);
+    if (mass < 128.0f) k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>(g);
+    else k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>(g);
+    // the tiles with a cell over 32 bits (none, mostly: the workgroups return at once)
+    const unsigned grid = nt < 256 ? nt : 256;
+    if (mass < 128.0f) k_deposit_wide<true><<<grid, DEP_NT, 0, s>>>(g);
+    else k_deposit_wide<false><<<grid, DEP_NT, 0, s>>>(g);
 }
 void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);

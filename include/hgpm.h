@@ -243,8 +243,10 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    arena (< 0 restores the full allocation of 6 32-bit cells per grid cell), "deposit_slots"
    = chunk descriptors per 16^3 owner block (1..64); tiles that do not fit take the
    atomic route, with identical results.  "deposit_deferred" (get only, synchronises)
-   = tiles that took the atomic route in the last deposit.  Unknown keys return
-   HGPM_ERR_ARG. */
+   = tiles that took the atomic route in the last deposit; "deposit_deferred_full" (get
+   only) = those of them that found their slot list or sub-arena full; "deposit_wide"
+   (get only) = tiles of the last deposit with a window cell over 32 bits, stored with
+   64-bit cells.  Unknown keys return HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
 /* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each

@@ -71,6 +71,7 @@ def test_deposit_routes(hg, orc, ng, np_):
         assert np.array_equal(m.get_density(), ref)
         d0 = m.get_tunable("deposit_deferred")
         assert d0 < ntiles  # most tiles took the arena route
+        slots0 = m.get_tunable("deposit_slots")
         for key, val in [("deposit_arena", 0), ("deposit_arena", 3 * 9000), ("deposit_slots", 1),
                          ("deposit_slots", 4)]:
             m.set_tunable(key, val)
@@ -82,7 +83,7 @@ def test_deposit_routes(hg, orc, ng, np_):
             elif val != 4:  # a 4-slot list may or may not overflow
                 assert 0 < d <= ntiles, (key, val, d)
             m.set_tunable("deposit_arena", -1)
-            m.set_tunable("deposit_slots", 64)
+            m.set_tunable("deposit_slots", slots0)
         m.cic()
         assert np.array_equal(m.get_density(), ref)
         # which tiles find arena room first depends on scheduling: only the bits are fixed
@@ -90,22 +91,45 @@ def test_deposit_routes(hg, orc, ng, np_):
 
 
 @pytest.mark.parametrize("np_", [64, 32])
-def test_deposit_dense_cells_take_the_atomic_route(hg, orc, np_):
+def test_deposit_dense_cells_take_the_wide_route(hg, orc, np_):
     """Tiles whose window holds a cell above 2^32 - 1 mass units (hundreds of particles
-    of one tile in one cell) cannot use the 32-bit arena: they are deferred to the
-    atomic route and the density stays bit-exact."""
+    of one tile in one cell) store their window again with 64-bit cells: nothing is
+    deferred and the density stays bit-exact.  With no arena room left they take the
+    atomic route instead.  A step over such tiles keeps the FP32 density handoff and
+    matches the step made of separate calls bit for bit."""
     ng = 64
     case(f"deposit dense cells ng={ng} np={np_}")
     rng = np.random.default_rng(77 + np_)
-    with mk(hg, ng, np_) as m:
+    p = hg.make_params(ng, np_, 2.0 * ng, 50.0, 0.0, 100)
+    with hg.MemoryManager(p) as m, hg.MemoryManager(p) as m2:
         x = (np.array([20.0, 33.0, 41.0]) + rng.normal(0, 0.3, (m.N, 3))) % ng
         cell = np.floor(x).astype(np.int32) % ng
         off = (x - np.floor(x)).astype(np.float32)
         off[off >= 1.0] = 0.0
-        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        vel = np.zeros((m.N, 3), np.float32)  # at rest: the first step deposits the clump
+        ref = orc.cic_split(ng, cell, off, (ng / np_) ** 3)
+        m.set_particles_split(cell, off, vel)
         m.cic()
-        assert np.array_equal(m.get_density(), orc.cic_split(ng, cell, off, (ng / np_) ** 3))
-        assert m.get_tunable("deposit_deferred") > 0
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_wide") > 0
+        assert m.get_tunable("deposit_deferred") == 0
+        m.set_tunable("deposit_arena", 100 * (m.N // 4096))  # room for part of the windows only
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_deferred_full") > 0
+        m.set_tunable("deposit_arena", -1)
+        m2.set_particles_split(cell, off, vel)
+        for s, co in enumerate(hg.schedule(p, 3, merge=True)):
+            m.step(co)
+            if s == 0:
+                assert m.get_tunable("deposit_wide") > 0 and m.get_tunable("deposit_deferred") == 0
+            if not co.skip_drift0:
+                m2.update_positions(co.drift0)
+            m2.cic()
+            m2.solve_gradient()
+            m2.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
+            for a, b in zip(m.get_particles_split(), m2.get_particles_split()):
+                assert np.array_equal(a, b)
 
 
 def test_deposit_mass_ratio(hg, orc):
