@@ -812,7 +812,7 @@ __device__ __forceinline__ float central_diff(float up, float down) { return up 
 // The budget must hold one base plane and row with their halo, 4 x 4 rows of ng + 3
 // cells padded to a multiple of 4, for ng <= 1024.
 constexpr int GT_SEGS = 2;                 // staged 4-cell segments per thread per load round
-constexpr int GT_GMAX = 16 * (1024 + 4);   // 64 KB of FP32 potential
+constexpr int GT_GMAX = 19968;             // 78 KB of FP32 potential (two workgroups per CU)
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
 
 // (registers for the two workgroups per CU the LDS budget allows: 4 waves per SIMD at
