@@ -814,6 +814,8 @@ __device__ __forceinline__ float central_diff(float up, float down) { return up 
 constexpr int GT_SEGS = 2;                 // staged 4-cell segments per thread per load round
 constexpr int GT_GMAX = 19968;             // 78 KB of FP32 potential (two workgroups per CU)
 constexpr int GT_HALO = 3;                 // staged cells beyond the base cells per axis
+constexpr int GT_MASKW = 64;               // staged x planes of a row-masked slab (at most)
+constexpr int GT_SPARSE = 4;               // staged cells per particle above which rows are masked
 
 // (registers for the two workgroups per CU the LDS budget allows: 4 waves per SIMD at
 // 512 threads, 2 at 256)
@@ -827,6 +829,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     static_assert(GMAX >= 16 * (1024 + 4), "one base plane and row of the widest window, padded rows");
     __shared__ __align__(16) float sp[GMAX];
     __shared__ int red[6][GT_NT / 64];
+    __shared__ unsigned long long msk[2 * GT_MASKW]; // row mask of a sparse slab: bm, need
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
     // the potential of this tile's box (a batch holds several boxes of the same shape)
@@ -898,20 +901,46 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
         const int nbx = min(tx, b[0] - sx0), nxs = nbx + GT_HALO;
         for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
             const int nby = min(ty, b[1] - sy0), nys = nby + GT_HALO;
-            int gxs, gys, gzs;
+            // The staged origin is one cell below the slab's first base cell on every axis.
+            const int bx0 = r0.cx + lo[0] + sx0 - 1, by0 = r0.cy + lo[1] + sy0 - 1, bz0 = r0.cz + lo[2] - 1;
+            // first base cell of the slab, wrapped (for the particles' offsets below)
+            const int gxs = wrap_cell(bx0 + 1, ng), gys = wrap_cell(by0 + 1, ng), gzs = wrap_cell(bz0 + 1, ng);
+            // Sparse slabs (late in a run the tile's particles fill their box thinly): the
+            // rows (staged x, y) that no particle reads are not staged.  bm[x] has bit y
+            // for each base column (x, y) of the slab's particles; a particle with base
+            // column (x, y) reads staged rows (x + 1 + u, y .. y + 3) for u = 0, 1 and
+            // (x, y + 1 .. y + 2), (x + 3, y + 1 .. y + 2): need[x] collects them.
+            const bool masked = nys <= 64 && nxs <= GT_MASKW && nxs * nys * ezp > GT_SPARSE * GT_NT * GT_PPT;
+            if (masked) {
+                unsigned long long* bm = msk;
+                unsigned long long* need = msk + GT_MASKW;
+                if (tid < GT_MASKW) bm[tid] = 0ull;
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < GT_PPT; k++) {
+                    int lx = P[k].cx - gxs, ly = P[k].cy - gys;
+                    lx += (lx >> 31) & ng;
+                    ly += (ly >> 31) & ng;
+                    if ((unsigned)lx < (unsigned)nbx && (unsigned)ly < (unsigned)nby) atomicOr(&bm[lx], 1ull << ly);
+                }
+                __syncthreads();
+                if (tid < nxs) {
+                    const unsigned long long b0 = tid < nbx ? bm[tid] : 0ull;
+                    const unsigned long long b1 = tid >= 1 && tid - 1 < nbx ? bm[tid - 1] : 0ull;
+                    const unsigned long long b2 = tid >= 2 && tid - 2 < nbx ? bm[tid - 2] : 0ull;
+                    const unsigned long long b3 = tid >= 3 && tid - 3 < nbx ? bm[tid - 3] : 0ull;
+                    const unsigned long long e2 = b0 | b3, e4 = b1 | b2;
+                    need[tid] = (e2 << 1) | (e2 << 2) | e4 | (e4 << 1) | (e4 << 2) | (e4 << 3);
+                }
+                __syncthreads();
+            }
             {
                 // Staging in 4-cell segments of the slab's rows (row = staged (x, y), ezp
                 // cells of z): item it -> row it / nseg, segment it % nseg.  Cells past ez
-                // in a row's last segment are padding, never read back.  The staged origin
-                // is one cell below the slab's first base cell on every axis.
-                const int bx0 = r0.cx + lo[0] + sx0 - 1, by0 = r0.cy + lo[1] + sy0 - 1, bz0 = r0.cz + lo[2] - 1;
+                // in a row's last segment are padding, never read back.
                 const int wz0 = wrap_cell(bz0, ng);
                 // the padded row inside one grid row (uniform per workgroup): no z wrap
                 const bool zin = wz0 + ezp <= ng;
-                // first base cell of the slab, wrapped (for the particles' offsets below)
-                gxs = wrap_cell(bx0 + 1, ng);
-                gys = wrap_cell(by0 + 1, ng);
-                gzs = wrap_cell(bz0 + 1, ng);
                 const int nitems = nxs * nys * nseg;
                 // it -> (row, segment) and row -> (x, y) by multiplication with rounded
                 // reciprocals, each quotient corrected by one step (exact for any sizes here)
@@ -919,6 +948,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 for (int i0 = tid; i0 < nitems; i0 += GT_SEGS * GT_NT) {
                     float a[GT_SEGS][4];
                     int at[GT_SEGS];
+                    bool use[GT_SEGS];
 #pragma unroll
                     for (int u = 0; u < GT_SEGS; u++) {
                         // items past the end load the slab's first segment (in bounds)
@@ -928,6 +958,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                         int x = (int)(((float)row + 0.5f) * rnys), y = row - x * nys;
                         if (y < 0) { x -= 1; y += nys; } else if (y >= nys) { x += 1; y -= nys; }
                         at[u] = row * ezp + 4 * sg;
+                        use[u] = i0 + u * GT_NT < nitems && (!masked || (msk[GT_MASKW + x] >> y & 1ull));
+                        if (!use[u]) continue;
                         int gx = bx0 + x, gy = by0 + y;
                         gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
                         gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
@@ -950,8 +982,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                     }
 #pragma unroll
                     for (int u = 0; u < GT_SEGS; u++)
-                        if (i0 + u * GT_NT < nitems)
-                            *(float4*)&sp[at[u]] = make_float4(a[u][0], a[u][1], a[u][2], a[u][3]);
+                        if (use[u]) *(float4*)&sp[at[u]] = make_float4(a[u][0], a[u][1], a[u][2], a[u][3]);
                 }
             }
             __syncthreads();
