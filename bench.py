@@ -200,6 +200,64 @@ def ensemble_boxes(hg, per_class=32, sizes=(64, 128, 192), z_ini=30.0, n_steps=4
     return ps
 
 
+def pressure_boxes(hg, n_big=24, n_small=200, z_ini=40.0, n_steps=30):
+    """The memory-pressure workload: n_big boxes of 384^3 particles on 768^3 grids (box
+    1500 Mpc/h) with n_small boxes of 96^3 particles on 96^3 grids (box 192 Mpc/h),
+    seeds 2000 + b, z_ini -> 0 in n_steps."""
+    ps = [hg.make_params(768, 384, 1500.0, z_ini, 0.0, n_steps, seed=2000 + b) for b in range(n_big)]
+    ps += [hg.make_params(96, 96, 192.0, z_ini, 0.0, n_steps, seed=2000 + n_big + b) for b in range(n_small)]
+    return ps
+
+
+def run_pressure(hg, steps, warmup, n_big, n_small, budget):
+    """The memory-pressure ensemble in one device budget (0: 90% of the free memory):
+    batches that do not fit keep grid buffers for a wave of their boxes and step the
+    waves in turn.  Then each size class alone in the same budget."""
+    ps = pressure_boxes(hg, n_big, n_small)
+    key = lambda p: (p.ng, p.np)
+    ics, scheds = {}, {}
+    for p in ps:
+        if key(p) not in ics:  # the classes share their cosmology; the seeds are per box
+            ics[key(p)] = ic_args(hg, p)
+            scheds[key(p)] = hg.schedule(p, warmup + steps, merge=True)
+
+    def one(pl):
+        with hg.Ensemble(pl, device_budget=budget) as e:
+            e.autotune(2)
+            t0 = time.perf_counter()
+            for b, p in enumerate(pl):
+                e.generate_ic(b, *ics[key(p)])
+            e.synchronize()
+            t_ic = time.perf_counter() - t0
+            for s in range(warmup):
+                e.step([scheds[key(p)][s] for p in pl])
+            e.synchronize()
+            t0 = time.perf_counter()
+            for s in range(warmup, warmup + steps):
+                e.step([scheds[key(p)][s] for p in pl])
+            e.synchronize()
+            el = time.perf_counter() - t0
+            nb, of = e.batches()
+            waves = [int(w) for w in e.waves()]
+            sizes = [int(sum(1 for x in of if x == g)) for g in range(nb)]
+            return {"ms_per_step": el * 1e3 / steps,
+                    "particle_steps_per_s": sum(p.np ** 3 for p in pl) * steps / el,
+                    "batches": [{"np": pl[list(of).index(g)].np, "ng": pl[list(of).index(g)].ng,
+                                 "boxes": sizes[g], "boxes_per_wave": waves[g]} for g in range(nb)],
+                    "device_GB": e.device_bytes() / 1e9, "ic_seconds": t_ic}
+
+    mixed = one(ps)
+    alone = [dict(one([p for p in ps if key(p) == k]), np=k[1], ng=k[0]) for k in sorted(ics)]
+    nparts = sum(p.np ** 3 for p in ps)
+    return {"metric": "aggregate particle-steps/s, ensemble under memory pressure",
+            "value": mixed["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": mixed["ms_per_step"], "higher_is_better": True,
+            "mixed": mixed, "per_class_alone": alone, "dtype": "f32",
+            "data": "synthetic: Zel'dovich ICs at z=40, Eisenstein-Hu no-wiggle P(k), per-box seed",
+            "config": {"workload": f"pressure_{n_big}x(384^3_on_768^3_box1500)+{n_small}x(96^3_box192)_z40",
+                       "boxes": len(ps), "particles": nparts, "device_budget_GB": budget / 1e9}}
+
+
 def ic_args(hg, p):
     ts = hg.Timestepper(p)
     ts.setInitialZ(p.z_ini)
@@ -314,6 +372,10 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched-ensemble workload instead (its own JSON line)")
     ap.add_argument("--per-class", type=int, default=32, help="ensemble boxes per size class")
+    ap.add_argument("--pressure", action="store_true",
+                    help="run the memory-pressure ensemble (24 x 384^3 on 768^3 with 200 x 96^3) instead")
+    ap.add_argument("--pressure-boxes", type=int, nargs=2, default=[24, 200], metavar=("BIG", "SMALL"))
+    ap.add_argument("--budget-gb", type=float, default=0.0, help="device budget of --pressure (0: 90%% free)")
     ap.add_argument("--only-copy", action="store_true",
                     help="run only the --copy-calibration copy (a separate profiler pass)")
     args = ap.parse_args()
@@ -332,6 +394,11 @@ def main():
         sys.exit(2)
     import hacc_gpm_amd as hg
 
+    if args.pressure:
+        steps = args.steps if args.steps is not None else max(1, 30 - args.warmup)
+        print(json.dumps(run_pressure(hg, steps, args.warmup, args.pressure_boxes[0], args.pressure_boxes[1],
+                                      int(args.budget_gb * 1e9))))
+        return
     if args.ensemble:
         steps = args.steps if args.steps is not None else max(1, 40 - args.warmup)
         print(json.dumps(run_ensemble(hg, steps, args.warmup, args.per_class)))

@@ -13,6 +13,7 @@ line with
   - ms_per_step of both paths and their ratio.
 
     python -m hacc_gpm_amd.accuracy --ng 512 --rl 1000 --z-ini 50 --steps 100 --seed 1
+    python -m hacc_gpm_amd.accuracy --ng 768 --np 384 --rl 1500 --z-ini 40 --steps 30 --seed 2000
 """
 import argparse
 import json
@@ -56,6 +57,7 @@ def run(m, sched, label):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--np", type=int, default=0, help="particles per side (default: ng)")
     ap.add_argument("--rl", type=float, default=1000.0)
     ap.add_argument("--z-ini", type=float, default=50.0)
     ap.add_argument("--steps", type=int, default=100)
@@ -63,7 +65,8 @@ def main():
     ap.add_argument("--pk-bins", type=int, default=221)
     args = ap.parse_args()
     ng = args.ng
-    mk = lambda prec: hg.make_params(ng, ng, args.rl, args.z_ini, 0.0, args.steps, seed=args.seed,
+    np_ = args.np or ng
+    mk = lambda prec: hg.make_params(ng, np_, args.rl, args.z_ini, 0.0, args.steps, seed=args.seed,
                                      pk_bins=args.pk_bins, precision=prec)
     p64, p32 = mk(hg.FP64), mk(hg.FP32)
     ts = hg.Timestepper(p64)
@@ -75,7 +78,7 @@ def main():
         fn = os.path.join(d, "eh.ipk")
         hg.write_ipk_eh(p64, fn, 100.0, 200001)
         hdr, vals = hg.read_ipk(fn)
-    out = {"workload": f"{ng}^3 particles, {ng}^3 grid, box {args.rl:g} Mpc/h, z={args.z_ini:g}->0, "
+    out = {"workload": f"{np_}^3 particles, {ng}^3 grid, box {args.rl:g} Mpc/h, z={args.z_ini:g}->0, "
                        f"{args.steps} steps, seed {args.seed}"}
     with hg.MemoryManager(p64) as m64, hg.MemoryManager(p32) as m32:
         hg.GenerateDisplacementIC(m64, p64, ts, hdr, vals)
