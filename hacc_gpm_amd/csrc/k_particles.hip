@@ -942,43 +942,60 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 // the padded row inside one grid row (uniform per workgroup): no z wrap
                 const bool zin = wz0 + ezp <= ng;
                 const int nitems = nxs * nys * nseg;
-                // it -> (row, segment) and row -> (x, y) by multiplication with rounded
-                // reciprocals, each quotient corrected by one step (exact for any sizes here)
-                const float rseg = 1.0f / (float)nseg, rnys = 1.0f / (float)nys;
-                for (int i0 = tid; i0 < nitems; i0 += GT_SEGS * GT_NT) {
+                // item it -> (row, segment), row -> (x, y): this thread's first items by
+                // division, then stepped by ST items a round; the step's quotients and
+                // remainders are uniform, so a step is a few adds with one carry each
+                constexpr int ST = GT_SEGS * GT_NT;
+                const int dsg = ST % nseg, drow = ST / nseg, dy = drow % nys, dx = drow / nys;
+                int sg[GT_SEGS], row[GT_SEGS], xs[GT_SEGS], ys[GT_SEGS];
+#pragma unroll
+                for (int u = 0; u < GT_SEGS; u++) {
+                    const int it = tid + u * GT_NT;
+                    row[u] = it / nseg;
+                    sg[u] = it - row[u] * nseg;
+                    xs[u] = row[u] / nys;
+                    ys[u] = row[u] - xs[u] * nys;
+                }
+                for (int i0 = tid; i0 < nitems; i0 += ST) {
                     float a[GT_SEGS][4];
                     int at[GT_SEGS];
                     bool use[GT_SEGS];
 #pragma unroll
                     for (int u = 0; u < GT_SEGS; u++) {
-                        // items past the end load the slab's first segment (in bounds)
-                        const int it = min(i0 + u * GT_NT, nitems - 1);
-                        int row = (int)(((float)it + 0.5f) * rseg), sg = it - row * nseg;
-                        if (sg < 0) { row -= 1; sg += nseg; } else if (sg >= nseg) { row += 1; sg -= nseg; }
-                        int x = (int)(((float)row + 0.5f) * rnys), y = row - x * nys;
-                        if (y < 0) { x -= 1; y += nys; } else if (y >= nys) { x += 1; y -= nys; }
-                        at[u] = row * ezp + 4 * sg;
+                        const int x = xs[u], y = ys[u];
+                        at[u] = row[u] * ezp + 4 * sg[u];
+                        // items past the end load nothing
                         use[u] = i0 + u * GT_NT < nitems && (!masked || (msk[GT_MASKW + x] >> y & 1ull));
-                        if (!use[u]) continue;
-                        int gx = bx0 + x, gy = by0 + y;
-                        gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-                        gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                        const float* __restrict rp = psi + ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng;
-                        if (zin) {
-                            const int z = wz0 + 4 * sg;
+                        if (use[u]) {
+                            int gx = bx0 + x, gy = by0 + y;
+                            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                            const float* __restrict rp =
+                                psi + ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng;
+                            if (zin) {
+                                const int z = wz0 + 4 * sg[u];
 #pragma unroll
-                            for (int c = 0; c < 4; c++) a[u][c] = rp[z + c];
-                        } else {
+                                for (int c = 0; c < 4; c++) a[u][c] = rp[z + c];
+                            } else {
 #pragma unroll
-                            for (int c = 0; c < 4; c++) {
-                                // ezp <= ng + 4 and wz0 < ng: one subtraction wraps (twice for
-                                // the padding of a row that spans the whole grid)
-                                int z = wz0 + 4 * sg + c;
-                                z = z >= ng ? z - ng : z;
-                                z = z >= ng ? z - ng : z;
-                                a[u][c] = rp[z];
+                                for (int c = 0; c < 4; c++) {
+                                    // ezp <= ng + 4 and wz0 < ng: one subtraction wraps (twice
+                                    // for the padding of a row that spans the whole grid)
+                                    int z = wz0 + 4 * sg[u] + c;
+                                    z = z >= ng ? z - ng : z;
+                                    z = z >= ng ? z - ng : z;
+                                    a[u][c] = rp[z];
+                                }
                             }
                         }
+                        sg[u] += dsg;
+                        const int cy = sg[u] >= nseg ? 1 : 0;
+                        sg[u] -= cy * nseg;
+                        row[u] += drow + cy;
+                        ys[u] += dy + cy;
+                        const int cx = ys[u] >= nys ? 1 : 0;
+                        ys[u] -= cx * nys;
+                        xs[u] += dx + cx;
                     }
 #pragma unroll
                     for (int u = 0; u < GT_SEGS; u++)
