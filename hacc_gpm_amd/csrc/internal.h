@@ -88,6 +88,8 @@ struct DepositBuffers {
     int nbox;    // boxes of the same shape stored back to back (particles, grids, owner blocks)
     unsigned long long* totals; // deposited mass per box
 };
+// The windows pass with 32-bit cells, then the 64-bit pass over the tiles it listed (a
+// cell over 2^32 - 1 mass units).  ctl must be cleared and the chunk counts zero.
 void launch_deposit_windows(const PosPackH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
 // rhof non-null: if no tile was deferred, write the FP32 density (q - mean) * 2^-24 to

@@ -143,7 +143,7 @@ struct DepRegs {
 // relative cell, e = extent + 1 (the upper CIC neighbour).  The two halves of the tile
 // (particles k < DEP_PPT/2 and k >= DEP_PPT/2 of every thread: the gather's tiles) get
 // their own boxes too: hb[h] = {lo[3], base cells per axis[3]}.
-__device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int64_t p0, int64_t n, int ng,
+__device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int64_t p0, int ng,
                                              const PosSplit& r0, DepRegs& R, int (*red)[DEP_NT / 64], int lo[3],
                                              int e[3], int (*hb)[6] = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -489,7 +489,7 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
     if (tid == 0) { sh.over = 0; sh.full = 0; } // ordered by the barrier in dep_load_box
     DepRegs R;
     int lo[3], e[3], hb[2][6];
-    dep_load_box(g.pos, p0, 0, ng, r0, R, sh.red, lo, e, hb);
+    dep_load_box(g.pos, p0, ng, r0, R, sh.red, lo, e, hb);
     if (!WIDE && tid == 0) {
         // the gather of these positions stages its potential window from these boxes.
         // Their origins are stored as absolute cells (centred into [-ng/2, ng/2)): the
@@ -763,7 +763,7 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosPack* __re
         const PosSplit r0 = unpack(pos[p0]);
         DepRegs R;
         int lo[3], e[3];
-        dep_load_box(pos, p0, n, ng, r0, R, red, lo, e);
+        dep_load_box(pos, p0, ng, r0, R, red, lo, e);
         unsigned long long* box_rho = rho + (int64_t)(tile / ntb) * gcells;
         const long long local = dep_window_atomic<SMALL>(win, R, lo, e, r0, ng, mass, box_rho);
         dep_tile_total(local, s_tot, &tile_total[tile]);
