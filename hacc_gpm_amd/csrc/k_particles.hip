@@ -216,6 +216,8 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
                                                              float mass) {
     unsigned long long dep = 0;
     const int ez = e[2], sxs = ny * ez;
+    const int lx = (int)(threadIdx.x >> 1) & 1, ly = (int)threadIdx.x & 1;
+    const int ox_[2] = {lx * sxs, (1 - lx) * sxs}, oy_[2] = {ly * ez, (1 - ly) * ez};
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
         // the weights are slab-invariant; an opaque copy of the offsets stops the
@@ -229,15 +231,23 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
         const int b0 = bx * sxs + by * ez + bz;
         // one particle's 8 weights sum to about mass * 2^24 < 2^31 when SMALL
         DepW<SMALL> pd = 0;
+        // step (jx, jy) adds the corner row (jx ^ lx, jy ^ ly): lanes whose particles
+        // share a cell (dense regions) then add to different rows at each step instead
+        // of queueing on one LDS address (alternating the z pair too cost more VALU
+        // than it saved)
+        const float hx[2] = {lx ? gx[1] : gx[0], lx ? gx[0] : gx[1]};
+        const float hy[2] = {ly ? gy[1] : gy[0], ly ? gy[0] : gy[1]};
 #pragma unroll
-        for (int x = 0; x < 2; x++) {
+        for (int jx = 0; jx < 2; jx++) {
+            const int x = jx ^ lx;
             if (!ALL && (bx + x < 0 || bx + x >= nx)) continue;
 #pragma unroll
-            for (int y = 0; y < 2; y++) {
+            for (int jy = 0; jy < 2; jy++) {
+                const int y = jy ^ ly;
                 if (!ALL && (by + y < 0 || by + y >= ny)) continue;
-                W* row = &win[b0 + x * sxs + y * ez];
-                const DepW<SMALL> q0 = fixed_weight_w<SMALL>(gx[x], gy[y], gz[0], mass);
-                const DepW<SMALL> q1 = fixed_weight_w<SMALL>(gx[x], gy[y], gz[1], mass);
+                W* row = &win[b0 + ox_[jx] + oy_[jy]];
+                const DepW<SMALL> q0 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[0], mass);
+                const DepW<SMALL> q1 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[1], mass);
                 pd += q0 + q1;
                 atomicAdd(row, (W)q0);
                 atomicAdd(row + 1, (W)q1);
