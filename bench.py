@@ -22,6 +22,7 @@ import argparse
 import json
 import os
 import sys
+import subprocess
 import tempfile
 import time
 
@@ -157,6 +158,28 @@ def run_full(hg, p, per_kernel=True):
             "per_kernel": per_kernel,
             "kernel_ms_first_10pct": region_ms(regions[:k]), "kernel_ms_last_10pct": region_ms(regions[-k:]),
             "ms_per_step": [round(1e3 * x, 3) for x in t]}
+
+
+def copy_ceiling(gib=8, reps=3):
+    """Streaming rate a plain elementwise kernel reaches on this device (torch
+    out = a + 0 over gib GiB, read + write): the practical ceiling beside the published
+    8 TB/s."""
+    import torch
+    n = (gib << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = 0.0
+    for _ in range(reps):
+        e0.record()
+        torch.add(a, 0.0, out=b)
+        e1.record()
+        torch.cuda.synchronize()
+        best = max(best, 2.0 * 4 * n / (e0.elapsed_time(e1) * 1e-3) / 1e12)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_TBps": best, "bytes_read": 4 * n, "bytes_written": 4 * n,
+            "note": "torch add(a, 0, out=b), best of %d" % reps}
 
 
 def cpu_baseline(hg, threads, ng=192):
@@ -376,9 +399,14 @@ def main():
                     help="run the memory-pressure ensemble (24 x 384^3 on 768^3 with 200 x 96^3) instead")
     ap.add_argument("--pressure-boxes", type=int, nargs=2, default=[24, 200], metavar=("BIG", "SMALL"))
     ap.add_argument("--budget-gb", type=float, default=0.0, help="device budget of --pressure (0: 90%% free)")
+    ap.add_argument("--only-ceiling", action="store_true",
+                    help="print the streaming copy rate of a plain elementwise kernel and exit")
     ap.add_argument("--only-copy", action="store_true",
                     help="run only the --copy-calibration copy (a separate profiler pass)")
     args = ap.parse_args()
+    if args.only_ceiling:
+        print(json.dumps(copy_ceiling()))
+        return
     if args.only_copy:
         import torch
         n = max(args.copy_calibration, 1) << 20
@@ -468,6 +496,12 @@ def main():
     }
     if not args.no_full_run:
         out["full_run"] = run_full(hg, p)
+        # torch and this library do not share one HIP runtime in a process: a child
+        # process measures the ceiling
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only-ceiling"], capture_output=True,
+                           text=True, timeout=300)
+        out["hbm_streaming_ceiling"] = (json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0
+                                        else {"error": r.stderr.strip()[-300:]})
     if not args.no_fp64:
         p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221, precision=hg.FP64)
         try:
