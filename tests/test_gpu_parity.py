@@ -156,6 +156,30 @@ def test_fft_fp32(hg, ng):
         assert rel_l2(back, ng ** 3 * x.astype(np.complex128)) < 4e-6
 
 
+@pytest.mark.parametrize("ng", [256, 384, 512, 768, 1024])
+def test_fft_fp32_separable(hg, ng):
+    """The larger plans (256 = 16*16, 384 = 3*8*16, 512 = 8*8*8, 768 = 3*16*16,
+    1024 = 16*8*8): a separable grid a(x) b(y) c(z) transforms to A(kx) B(ky) C(kz), with
+    A, B, C the 1-D transforms (numpy, FP64), checked at sampled modes; the inverse
+    returns ng^3 times the grid."""
+    case(f"fft fp32 separable ng={ng}")
+    rng = np.random.default_rng(ng + 5)
+    v = [(rng.standard_normal(ng) + 1j * rng.standard_normal(ng)).astype(np.complex64) for _ in range(3)]
+    x = (v[0][:, None] * v[1][None, :])[:, :, None] * v[2][None, None, :]
+    V = [np.fft.fft(u.astype(np.complex128)) for u in v]
+    idx = rng.integers(0, ng, (200000, 3))
+    with mk(hg, ng, 32) as m:
+        X = m.fft(x, -1)
+        got = X[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.complex128)
+        want = V[0][idx[:, 0]] * V[1][idx[:, 1]] * V[2][idx[:, 2]]
+        assert rel_l2(got, want) < 5e-6
+        back = m.fft(X, +1)
+        del X
+        got = back[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.complex128)
+        want = float(ng) ** 3 * x[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.complex128)
+        assert rel_l2(got, want) < 1e-5
+
+
 @pytest.mark.parametrize("ng", [32, 64, 96])
 def test_fft_fp64(hg, ng):
     case(f"fft fp64 ng={ng}")
