@@ -209,6 +209,31 @@ def test_solve_fp32_vs_fp64_oracle(hg, orc, ng):
     assert err < 2e-5, err
 
 
+@pytest.mark.parametrize("ng", [256, 512, 768, 1024])
+def test_solve_fp32_vs_fp64_path_large(hg, ng):
+    """The FP32 potential solve at the larger plans (half spectra: forward z, y, fused
+    x pass, inverse y, inverse z; forces as central differences of the potential)
+    against the FP64 plain path's k-space gradient of the same deposit: rel L2 < 2e-5,
+    the bound the FP64 oracle check uses at small sizes."""
+    case(f"solve fp32 vs fp64 path ng={ng}")
+    rng = np.random.default_rng(ng + 9)
+    np_ = ng // 2
+    with mk(hg, ng, np_) as m:
+        cell, off = lattice_split(rng, np_, ng, 2.0)
+        m.set_particles_split(cell, off, np.zeros((np_ ** 3, 3), np.float32))
+        del cell, off
+        m.cic()
+        q = m.get_density()
+        m.solve_gradient()
+        F = m.get_force()
+    with mk(hg, ng, 32, prec=1) as m64:
+        m64.set_density_f64(q.astype(np.float64) * 2.0 ** -24)
+        del q
+        m64.solve_gradient()
+        F64 = m64.get_force()
+    assert rel_l2(F.astype(np.float64), F64) < 2e-5
+
+
 @pytest.mark.parametrize("ng", [32, 96, 128])
 def test_fft_variants_agree(hg, orc, ng):
     """Every column-pass / x-pass variant the autotuner may pick gives the same physics."""
