@@ -55,10 +55,11 @@ def test_deposit_bit_exact(hg, orc, ng, clustered):
         assert np.array_equal(m.get_density(), got)
 
 
-@pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64)])
+@pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64), (256, 256)])
 def test_deposit_routes(hg, orc, ng, np_):
     """Arena route (windows -> owner blocks), atomic route for deferred tiles, and the
-    mixes forced by a short arena or a short slot list: all bit-identical."""
+    mixes forced by a short arena or a short slot list: all bit-identical.  256^3: 4096
+    tiles over 16 sub-arenas."""
     case(f"deposit routes ng={ng} np={np_}")
     rng = np.random.default_rng(ng * 7 + np_)
     mass = (ng / np_) ** 3
