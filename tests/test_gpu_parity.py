@@ -283,11 +283,12 @@ def test_solve_fp64_path(hg, orc, ng):
 
 # --------------------------------------------------------- kick / drift ----
 @pytest.mark.parametrize("ng,np_,dist", [(32, 32, "random"), (64, 64, "random"), (64, 64, "lattice"),
-                                         (128, 64, "lattice"), (64, 64, "sparse")])
+                                         (128, 64, "lattice"), (64, 64, "sparse"), (512, 32, "lattice")])
 def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
     """Kick and drift against the oracle, bit for bit.  "sparse": displacements of 1.5
     cells fill each gather tile's box thinly, so the gather stages only the rows its
-    particles read (row-masked slabs)."""
+    particles read (row-masked slabs).  (512, 32): particles 16 cells apart, so a tile's
+    box spans hundreds of cells and is staged in many slabs."""
     case(f"kick/drift ng={ng} np={np_} {dist}")
     rng = np.random.default_rng(ng + 11)
     with mk(hg, ng, np_) as m:
