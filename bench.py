@@ -81,6 +81,11 @@ def make_ics(hg, p, ts):
     return hdr, vals
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_engine(hg, p, steps, warmup, timing=True, gather_tile=-1):
     ts = hg.Timestepper(p)
     ts.setInitialZ(p.z_ini)
@@ -269,7 +274,9 @@ def run_pressure(hg, steps, warmup, n_big, n_small, budget):
                                  "boxes": sizes[g], "boxes_per_wave": waves[g]} for g in range(nb)],
                     "device_GB": e.device_bytes() / 1e9, "ic_seconds": t_ic}
 
+    progress(f"{len(ps)} boxes in one ensemble")
     mixed = one(ps)
+    progress("each size class alone")
     alone = [dict(one([p for p in ps if key(p) == k]), np=k[1], ng=k[0]) for k in sorted(ics)]
     nparts = sum(p.np ** 3 for p in ps)
     return {"metric": "aggregate particle-steps/s, ensemble under memory pressure",
@@ -330,6 +337,7 @@ def run_ensemble(hg, steps, warmup, per_class):
         e.run([[sc[s] for sc in scheds] for s in range(warmup, n_total)])
         e.synchronize()
         t_batched = time.perf_counter() - t0
+    progress("one box at a time")
     t_single = 0.0
     for b, p in enumerate(ps):
         with hg.MemoryManager(p) as m:
@@ -343,6 +351,7 @@ def run_ensemble(hg, steps, warmup, per_class):
                 m.synchronize()
             t_single += time.perf_counter() - t0
     # each size class batched on its own (no other class sharing the GPU)
+    progress("each size class alone")
     classes = []
     for n in sorted({p.np for p in ps}):
         idx = [b for b, p in enumerate(ps) if p.np == n]
@@ -439,7 +448,9 @@ def main():
     p = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221)
     Np = ng ** 3
     G = ng ** 3
+    progress(f"{workload}: {args.warmup} warm-up and {args.steps} timed steps")
     el, tim, t_ic, dev_bytes, tune = run_engine(hg, p, args.steps, args.warmup, gather_tile=args.gather_tile)
+    progress(f"timed steps done: {el * 1e3 / args.steps:.2f} ms per step")
     value = Np * args.steps / el
     ms_step = el * 1e3 / args.steps
     # dominant kernel region
@@ -495,7 +506,9 @@ def main():
         "device_GB": dev_bytes / 1e9,
     }
     if not args.no_full_run:
+        progress("whole run, step by step")
         out["full_run"] = run_full(hg, p)
+        progress("streaming ceiling (child process)")
         # torch and this library do not share one HIP runtime in a process: a child
         # process measures the ceiling
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--only-ceiling"], capture_output=True,
@@ -505,6 +518,7 @@ def main():
     if not args.no_fp64:
         p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221, precision=hg.FP64)
         try:
+            progress("FP64 plain path")
             el64, _, _, b64, _ = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
             v64 = Np * args.fp64_steps / el64
             out["fp64_plain"] = {"value": v64, "ms_per_step": el64 * 1e3 / args.fp64_steps,
@@ -514,6 +528,7 @@ def main():
             out["fp64_plain"] = {"error": str(e)}
     if not args.no_cpu:
         th = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        progress(f"CPU baseline on {th} threads")
         out["cpu_baseline"] = cpu_baseline(hg, th)
     if args.copy_calibration:
         import torch
