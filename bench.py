@@ -132,6 +132,7 @@ def run_full(hg, p, per_kernel=True):
     sched = hg.schedule(p, n, merge=True)
     t = []
     deferred = []
+    windows = []  # arena cells reserved per grid cell, per step
     regions = []
     with hg.MemoryManager(p) as m:
         m.autotune(3)
@@ -147,6 +148,7 @@ def run_full(hg, p, per_kernel=True):
             regions.append(m.timings() if per_kernel else {})
             deferred.append([m.get_tunable(k) for k in
                              ("deposit_deferred", "deposit_wide", "deposit_deferred_full")])
+            windows.append(m.get_tunable("deposit_window_mcells") / (p.ng ** 3 / 2 ** 20))
     k = max(1, n // 10)
     Np = p.np ** 3
 
@@ -160,6 +162,7 @@ def run_full(hg, p, per_kernel=True):
             # per step: tiles deferred, tiles with 64-bit windows, tiles deferred for a full
             # slot list or sub-arena
             "deferred_tiles": deferred,
+            "window_cells_per_grid_cell": [round(w, 3) for w in windows],
             "per_kernel": per_kernel,
             "kernel_ms_first_10pct": region_ms(regions[:k]), "kernel_ms_last_10pct": region_ms(regions[-k:]),
             "ms_per_step": [round(1e3 * x, 3) for x in t]}

@@ -1191,6 +1191,17 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
     if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
+    if (!strcmp(key, "deposit_window_mcells")) { // arena cells the last deposit reserved, in units of 2^20
+        if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
+        std::vector<unsigned long long> v(HGPM_DEP_CTL);
+        HIPCHK(c, hipMemcpyAsync(v.data(), c->ctl, sizeof(unsigned long long) * HGPM_DEP_CTL, hipMemcpyDeviceToHost,
+                                 c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        unsigned long long t = 0;
+        for (int s = 4; s < HGPM_DEP_CTL; s++) t += v[s];
+        *value = (int)(t >> 20);
+        return HGPM_OK;
+    }
     // last deposit: tiles that took the atomic route, tiles whose windows were stored
     // with 64-bit cells, deferred tiles that found their slot list or sub-arena full
     const char* dkeys[3] = {"deposit_deferred", "deposit_wide", "deposit_deferred_full"};

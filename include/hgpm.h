@@ -246,7 +246,8 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    = tiles that took the atomic route in the last deposit; "deposit_deferred_full" (get
    only) = those of them that found their slot list or sub-arena full; "deposit_wide"
    (get only) = tiles of the last deposit with a window cell over 32 bits, stored with
-   64-bit cells.  Unknown keys return HGPM_ERR_ARG. */
+   64-bit cells; "deposit_window_mcells" (get only) = arena cells the last deposit
+   reserved for its windows, in units of 2^20.  Unknown keys return HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
 /* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each
