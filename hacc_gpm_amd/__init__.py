@@ -86,6 +86,7 @@ def lib():
         sig = {
             "hgpm_version": (c_char_p, []),
             "hgpm_last_error": (c_char_p, [vp]),
+            "hgpm_device_count": (c_int, [P(c_int)]),
             "hgpm_read_params": (c_int, [c_char_p, P(Params)]),
             "hgpm_save_params": (c_int, [P(Params)]),
             "hgpm_read_ipk": (c_int, [c_char_p, vp, vp, c_int, P(c_int)]),
@@ -108,6 +109,8 @@ def lib():
             "hgpm_get_particles_split": (c_int, [vp, vp, vp, vp]),
             "hgpm_set_particles_f64": (c_int, [vp, vp, vp]),
             "hgpm_get_particles_f64": (c_int, [vp, vp, vp]),
+            "hgpm_copy_particles": (c_int, [vp, vp]),
+            "hgpm_compare_particles": (c_int, [vp, vp, vp]),
             "hgpm_generate_ic": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double, c_double]),
             "hgpm_update_positions": (c_int, [vp, c_double]),
             "hgpm_cic": (c_int, [vp]),
@@ -178,6 +181,14 @@ def _ptr(a):
 
 def version():
     return lib().hgpm_version().decode()
+
+
+def device_count():
+    """HIP devices visible to this process (0 without a GPU), asked of libhgpm's own HIP
+    runtime, so no other runtime is initialised in this process."""
+    n = ctypes.c_int(0)
+    _check(lib().hgpm_device_count(ctypes.byref(n)))
+    return n.value
 
 
 # ----------------------------------------------------------- host side ----
@@ -347,6 +358,19 @@ class MemoryManager:
         vel = np.zeros((self.N, 3), np.float64)
         self._c(lib().hgpm_get_particles_f64(self.h, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def copy_particles_from(self, other):
+        """Particle state of another context (same NP, NG), converted on the device."""
+        self._c(lib().hgpm_copy_particles(self.h, other.h))
+
+    def compare_particles(self, other):
+        """RMS periodic position difference (grid units), velocity RMS difference
+        relative to other's RMS velocity, and the largest position difference of a
+        component, against another context of the same lattice (FP64 sums on the device)."""
+        out = np.zeros(4)
+        self._c(lib().hgpm_compare_particles(self.h, other.h, _ptr(out)))
+        return {"rms_disp_cells": float(np.sqrt(out[0] / self.N)), "vel_rel_rms": float(np.sqrt(out[1] / out[2]))
+                if out[2] > 0 else 0.0, "max_disp_cells": float(out[3]), "sums": out.tolist()}
 
     # steps
     def generate_ic(self, ipk_header, ipk_values, delta, dot_delta, fscal, deltaT):

@@ -395,6 +395,17 @@ const char* hgpm_version(void) { return "hgpm 0.1.0 (gfx950)"; }
 
 const char* hgpm_last_error(const hgpm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
 
+int hgpm_device_count(int* n) {
+    if (!n) return fail(nullptr, HGPM_ERR_ARG, "hgpm_device_count: null argument");
+    int d = 0;
+    if (hipGetDeviceCount(&d) != hipSuccess) {
+        (void)hipGetLastError();
+        d = 0;
+    }
+    *n = d;
+    return HGPM_OK;
+}
+
 namespace {
 int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
     if (!p || !out) return fail(nullptr, HGPM_ERR_ARG, "hgpm_create: null argument");
@@ -476,7 +487,10 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
                 (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
                 (rc = dalloc(c, (char**)&c->boxes, 64 * ntiles, "gather boxes")))
                 return bail(rc);
-            if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess) {
+            // particles start at cell 0, offset 0 (valid if never set; a step refuses them)
+            if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess ||
+                hipMemset(c->pos, 0, sizeof(PosPackH) * NB) != hipSuccess ||
+                hipMemset(c->vel, 0, sizeof(float) * 3 * NB) != hipSuccess) {
                 c->err = "hipMemset failed";
                 return bail(HGPM_ERR_HIP);
             }
@@ -572,6 +586,11 @@ int hgpm_set_particles(hgpm_ctx* c, const float* pos, const float* vel) {
     const size_t n3 = 3 * (size_t)c->N;
     c->boxes_valid = false;
     if (c->prec == HGPM_FP32) {
+        // the device split (carry_wrap) converts floor(x) to int: reject what it cannot
+        // hold, and NaN / Inf, before anything reaches the device
+        for (size_t i = 0; i < n3; i++)
+            if (!(std::fabs(pos[i]) < 1073741824.0f) || !std::isfinite(vel[i]))
+                return fail(c, HGPM_ERR_ARG, "non-finite particle (or |x| >= 2^30) at index " + std::to_string(i / 3));
         float* tmp = nullptr;
         int rc = dalloc(c, &tmp, sizeof(float) * 2 * n3, "staging");
         if (rc) return rc;
@@ -640,6 +659,8 @@ int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off,
             float f = off[3 * i + d];
             if (cc < 0 || cc >= c->ng || !(f >= 0.0f && f < 1.0f))
                 return fail(c, HGPM_ERR_ARG, "split particle out of range at index " + std::to_string(i));
+            if (!std::isfinite(vel[3 * i + d]))
+                return fail(c, HGPM_ERR_ARG, "non-finite velocity at index " + std::to_string(i));
         }
         const unsigned pc = ((unsigned)cell[3 * i] << (2 * HGPM_CELL_BITS_H)) |
                             ((unsigned)cell[3 * i + 1] << HGPM_CELL_BITS_H) | (unsigned)cell[3 * i + 2];
@@ -719,6 +740,76 @@ int hgpm_get_particles_f64(hgpm_ctx* c, double* pos, double* vel) {
     HIPCHK(c, hipMemcpyAsync(pos, c->pos64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(vel, c->vel64, sizeof(double) * n3, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    return HGPM_OK;
+}
+
+// ------------------------------------------- particle state between contexts ----
+namespace {
+int pair_check(hgpm_ctx* a, hgpm_ctx* b, const char* what) {
+    if (!a || !b) return fail(a ? a : b, HGPM_ERR_ARG, std::string(what) + ": null context");
+    if (a->np != b->np || a->ng != b->ng || a->device != b->device)
+        return fail(a, HGPM_ERR_ARG, std::string(what) + ": contexts differ in NP, NG or device");
+    if (a->nstore != 1 || b->nstore != 1)
+        return fail(a, HGPM_ERR_UNSUPPORTED, std::string(what) + ": single-box contexts only");
+    return HGPM_OK;
+}
+} // namespace
+
+int hgpm_copy_particles(hgpm_ctx* dst, hgpm_ctx* src) {
+    int rc = pair_check(dst, src, "hgpm_copy_particles");
+    if (rc) return rc;
+    if ((rc = need_particles(src))) return fail(dst, rc, src->err);
+    HIPCHK(dst, hipStreamSynchronize(src->st));
+    const size_t N = (size_t)dst->N;
+    if (src->prec == HGPM_FP64 && dst->prec == HGPM_FP32) {
+        launch_split_from_f64(src->pos64, src->vel64, dst->pos, dst->vel, dst->N, dst->np, dst->ng, dst->st);
+        LAUNCHCHK(dst);
+    } else if (src->prec == HGPM_FP32 && dst->prec == HGPM_FP64) {
+        launch_split_to_f64(src->pos, src->vel, dst->pos64, dst->vel64, dst->N, dst->np, dst->st);
+        LAUNCHCHK(dst);
+    } else if (src->prec == HGPM_FP32) {
+        HIPCHK(dst, hipMemcpyAsync(dst->pos, src->pos, sizeof(PosPackH) * N, hipMemcpyDeviceToDevice, dst->st));
+        HIPCHK(dst, hipMemcpyAsync(dst->vel, src->vel, sizeof(float) * 3 * N, hipMemcpyDeviceToDevice, dst->st));
+    } else {
+        HIPCHK(dst, hipMemcpyAsync(dst->pos64, src->pos64, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, dst->st));
+        HIPCHK(dst, hipMemcpyAsync(dst->vel64, src->vel64, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, dst->st));
+    }
+    HIPCHK(dst, hipStreamSynchronize(dst->st));
+    dst->boxes_valid = false;
+    dst->have_particles = true;
+    dst->have_density = dst->have_force = false;
+    return HGPM_OK;
+}
+
+int hgpm_compare_particles(hgpm_ctx* a, hgpm_ctx* b, double* out) {
+    int rc = pair_check(a, b, "hgpm_compare_particles");
+    if (rc) return rc;
+    if (!out) return fail(a, HGPM_ERR_ARG, "hgpm_compare_particles: null output");
+    if ((rc = need_particles(a))) return rc;
+    if ((rc = need_particles(b))) return fail(a, rc, b->err);
+    HIPCHK(a, hipStreamSynchronize(b->st));
+    const int nw = compare_partials();
+    double* part = nullptr;
+    if ((rc = dalloc(a, &part, sizeof(double) * 4 * nw, "compare partials"))) return rc;
+    a->bytes -= sizeof(double) * 4 * nw;
+    const bool a64 = a->prec == HGPM_FP64, b64 = b->prec == HGPM_FP64;
+    launch_compare(a64 ? nullptr : a->pos, a64 ? nullptr : a->vel, a64 ? a->pos64 : nullptr, a64 ? a->vel64 : nullptr,
+                   b64 ? nullptr : b->pos, b64 ? nullptr : b->vel, b64 ? b->pos64 : nullptr, b64 ? b->vel64 : nullptr,
+                   a->N, a->np, a->ng, part, a->st);
+    hipError_t le = hipGetLastError();
+    std::vector<double> h(4 * (size_t)nw);
+    hipError_t ce = le == hipSuccess ? hipMemcpyAsync(h.data(), part, sizeof(double) * 4 * nw, hipMemcpyDeviceToHost, a->st)
+                                     : le;
+    if (ce == hipSuccess) ce = hipStreamSynchronize(a->st);
+    (void)hipFree(part);
+    if (ce != hipSuccess) return fail(a, HGPM_ERR_HIP, std::string("hgpm_compare_particles: ") + hipGetErrorString(ce));
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    for (int w = 0; w < nw; w++) {
+        out[0] += h[4 * w];
+        out[1] += h[4 * w + 1];
+        out[2] += h[4 * w + 2];
+        out[3] = std::max(out[3], h[4 * w + 3]);
+    }
     return HGPM_OK;
 }
 
@@ -1191,7 +1282,10 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
     if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
-    if (!strcmp(key, "deposit_window_mcells")) { // arena cells the last deposit reserved, in units of 2^20
+    if (!strcmp(key, "deposit_arena_mwords")) {
+        // 32-bit arena words the last deposit reserved, in units of 2^20: the windows'
+        // reservations, those of tiles then found full included, plus the doubled ones
+        // of the 64-bit pass (an upper bound of the window volume, not the volume)
         if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
         std::vector<unsigned long long> v(HGPM_DEP_CTL);
         HIPCHK(c, hipMemcpyAsync(v.data(), c->ctl, sizeof(unsigned long long) * HGPM_DEP_CTL, hipMemcpyDeviceToHost,
@@ -1279,6 +1373,7 @@ struct hgpm_ensemble {
     std::vector<int> batch_of, slot_of; // per box
     std::vector<int> first;             // per batch: its first box
     std::vector<hgpm_params> params;    // per box
+    std::vector<char> ready;            // per box: particles set or generated
     std::string err;
     // hgpm_ensemble_run: one event per batch stream (fork / join / done) and the
     // launched graphs, each with the event that marks its end, freed once it passed.
@@ -1370,6 +1465,20 @@ int ebox(hgpm_ensemble* e, int box) {
     if (box < 0 || box >= (int)e->params.size()) return efail(e, HGPM_ERR_ARG, "box index out of range");
     return HGPM_OK;
 }
+// A batch shares one have_particles flag; each box must have had its particles set or
+// generated before anything reads them.
+int eready(hgpm_ensemble* e, int box) {
+    if (!e->ready[box])
+        return efail(e, HGPM_ERR_STATE, "box " + std::to_string(box) + " has no particles (set or generate them first)");
+    return HGPM_OK;
+}
+int eall_ready(hgpm_ensemble* e) {
+    for (size_t b = 0; b < e->ready.size(); b++) {
+        int rc = eready(e, (int)b);
+        if (rc) return rc;
+    }
+    return HGPM_OK;
+}
 
 // Forwards a context failure of box b's batch to the ensemble's message.
 int eforward(hgpm_ensemble* e, hgpm_ctx* c, int rc) {
@@ -1392,6 +1501,7 @@ int hgpm_ensemble_create_budget(int n_boxes, const hgpm_params* params, size_t d
     e->params.assign(params, params + n_boxes);
     e->batch_of.assign(n_boxes, -1);
     e->slot_of.assign(n_boxes, -1);
+    e->ready.assign(n_boxes, 0);
     std::vector<std::vector<int>> members;
     for (int b = 0; b < n_boxes; b++) {
         if (params[b].precision != HGPM_FP32) {
@@ -1502,7 +1612,9 @@ int hgpm_ensemble_generate_ic(hgpm_ensemble* e, int box, const double* ipk, int 
     if (rc) return rc;
     hgpm_ctx* c = e->batch[e->batch_of[box]];
     BoxView v(c, e->slot_of[box], e->params[box]);
-    return eforward(e, c, hgpm_generate_ic(c, ipk, ipk_bins, ipk_k_delta, ipk_k_max, delta, dot_delta, fscal, deltaT));
+    rc = eforward(e, c, hgpm_generate_ic(c, ipk, ipk_bins, ipk_k_delta, ipk_k_max, delta, dot_delta, fscal, deltaT));
+    if (rc == HGPM_OK) e->ready[box] = 1;
+    return rc;
 }
 
 int hgpm_ensemble_set_particles(hgpm_ensemble* e, int box, const float* pos, const float* vel) {
@@ -1510,12 +1622,14 @@ int hgpm_ensemble_set_particles(hgpm_ensemble* e, int box, const float* pos, con
     if (rc) return rc;
     hgpm_ctx* c = e->batch[e->batch_of[box]];
     BoxView v(c, e->slot_of[box], e->params[box]);
-    return eforward(e, c, hgpm_set_particles(c, pos, vel));
+    rc = eforward(e, c, hgpm_set_particles(c, pos, vel));
+    if (rc == HGPM_OK) e->ready[box] = 1;
+    return rc;
 }
 
 int hgpm_ensemble_get_particles(hgpm_ensemble* e, int box, float* pos, float* vel) {
     int rc = ebox(e, box);
-    if (rc) return rc;
+    if (rc || (rc = eready(e, box))) return rc;
     hgpm_ctx* c = e->batch[e->batch_of[box]];
     BoxView v(c, e->slot_of[box], e->params[box]);
     return eforward(e, c, hgpm_get_particles(c, pos, vel));
@@ -1523,7 +1637,7 @@ int hgpm_ensemble_get_particles(hgpm_ensemble* e, int box, float* pos, float* ve
 
 int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k, double* pk, int64_t* count) {
     int rc = ebox(e, box);
-    if (rc) return rc;
+    if (rc || (rc = eready(e, box))) return rc;
     hgpm_ctx* c = e->batch[e->batch_of[box]];
     BoxView v(c, e->slot_of[box], e->params[box]);
     return eforward(e, c, hgpm_power_spectrum(c, fold, k, pk, count));
@@ -1534,6 +1648,7 @@ int hgpm_ensemble_power_spectrum(hgpm_ensemble* e, int box, int fold, double* k,
 int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs) {
     if (!e || !coeffs) return efail(e, HGPM_ERR_ARG, "hgpm_ensemble_step: null argument");
     if (check_coeffs(e, coeffs)) return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch");
+    if (int rc = eall_ready(e)) return rc;
     for (size_t g = 0; g < e->batch.size(); g++) {
         int rc = hgpm_step(e->batch[g], &coeffs[e->first[g]]);
         if (rc) return eforward(e, e->batch[g], rc);
@@ -1553,10 +1668,7 @@ int hgpm_ensemble_run(hgpm_ensemble* e, int nsteps, const hgpm_step_coeffs* coef
     for (int s = 0; s < nsteps; s++)
         if (check_coeffs(e, coeffs + (size_t)s * nb))
             return efail(e, HGPM_ERR_ARG, "step coefficients differ within a batch (step " + std::to_string(s) + ")");
-    for (hgpm_ctx* c : e->batch) {
-        int rc = need_particles(c);
-        if (rc) return eforward(e, c, rc);
-    }
+    if (int rc = eall_ready(e)) return rc;
     if (nsteps == 0) return HGPM_OK;
     release_graphs(e, false);
     while (e->ev.size() < ng) {

@@ -119,6 +119,18 @@ void launch_fold_split(PosPackH* pos, int64_t np3, int ng, hipStream_t s);
 // forces (3 floats per cell) of one box from its potential, as the gather forms them
 void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s);
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s);
+// FP64 reference-order double3 <-> FP32 split layout in internal order (one box)
+void launch_split_from_f64(const double* p3, const double* v3, PosPackH* pos, float* vel, int64_t n, int np, int ng,
+                           hipStream_t s);
+void launch_split_to_f64(const PosPackH* pos, const float* vel, double* p3, double* v3, int64_t n, int np,
+                         hipStream_t s);
+// Differences of two particle states (each given as split pos/vel, or as pos64/vel64 with
+// the split pointers null): compare_partials() workgroups write 4 doubles each to
+// partial: sum |dx|^2 (periodic, grid units), sum |dv|^2, sum |v_b|^2, max |dx| component.
+int compare_partials();
+void launch_compare(const PosPackH* pa, const float* va, const double* pa64, const double* va64, const PosPackH* pb,
+                    const float* vb, const double* pb64, const double* vb64, int64_t n, int np, int ng,
+                    double* partial, hipStream_t s);
 
 // ---- P(k) (k_power.hip) ----
 // bins of a complex spectrum (ng^3) with FP64 deterministic sums.

@@ -1135,6 +1135,110 @@ __global__ void k_to_reference(const PosPack* __restrict src, const float* __res
     vdst[3 * i] = vsrc[3 * j]; vdst[3 * i + 1] = vsrc[3 * j + 1]; vdst[3 * i + 2] = vsrc[3 * j + 2];
 }
 
+// double3 grid-unit positions in reference order (the FP64 path's layout) -> the split
+// layout in internal order: c = floor(x), f = float(x - c) (an f that rounds to 1
+// carries into the cell), velocities rounded to FP32.
+__global__ void k_split_from_f64(const double* __restrict p3, const double* __restrict v3, PosPack* __restrict pos,
+                                 float* __restrict vel, int64_t n, int np, int ng) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    int c[3];
+    float f[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double x = p3[3 * i + d];
+        const double fl = floor(x);
+        float r = (float)(x - fl);
+        int cc = (int)fl;
+        if (r >= 1.0f) { r = 0.0f; cc += 1; }
+        cc %= ng;
+        if (cc < 0) cc += ng;
+        c[d] = cc;
+        f[d] = r;
+        vel[3 * j + d] = (float)v3[3 * i + d];
+    }
+    pos[j] = pack(PosSplit{c[0], c[1], c[2], f[0], f[1], f[2]});
+}
+// ... and back: x = (double)c + (double)f, velocities widened.
+__global__ void k_split_to_f64(const PosPack* __restrict pos, const float* __restrict vel, double* __restrict p3,
+                               double* __restrict v3, int64_t n, int np) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int qx, qy, qz;
+    const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+    const PosSplit p = unpack(pos[j]);
+    p3[3 * i] = (double)p.cx + (double)p.fx;
+    p3[3 * i + 1] = (double)p.cy + (double)p.fy;
+    p3[3 * i + 2] = (double)p.cz + (double)p.fz;
+    v3[3 * i] = vel[3 * j]; v3[3 * i + 1] = vel[3 * j + 1]; v3[3 * i + 2] = vel[3 * j + 2];
+}
+
+// Differences of two particle states of the same lattice (a, b: FP32 split layout in
+// internal order, or FP64 double3 in reference order): per workgroup, in FP64, the sums
+// of the squared periodic position differences (grid units), of the squared velocity
+// differences and of b's squared velocities, and the largest position difference of a
+// component.  Fixed grid and fixed-order reductions: the same bits on every run.
+struct ParticleView {
+    const PosPack* pos;
+    const float* vel;
+    const double* pos64;
+    const double* vel64;
+};
+constexpr int CMP_NT = 256;
+constexpr int CMP_WG = 1024;
+template <bool F64>
+__device__ __forceinline__ void view_particle(const ParticleView& v, int64_t j, int64_t i, double x[3], double u[3]) {
+    if constexpr (F64) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) { x[d] = v.pos64[3 * i + d]; u[d] = v.vel64[3 * i + d]; }
+    } else {
+        const PosSplit p = unpack(v.pos[j]);
+        x[0] = (double)p.cx + (double)p.fx;
+        x[1] = (double)p.cy + (double)p.fy;
+        x[2] = (double)p.cz + (double)p.fz;
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[d] = v.vel[3 * j + d];
+    }
+}
+template <bool A64, bool B64>
+__global__ __launch_bounds__(CMP_NT) void k_compare(ParticleView a, ParticleView b, int64_t n, int np, int ng,
+                                                    double* __restrict partial) {
+    __shared__ double red[4][CMP_NT];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    const double g = (double)ng;
+    for (int64_t j = (int64_t)blockIdx.x * CMP_NT + threadIdx.x; j < n; j += (int64_t)CMP_WG * CMP_NT) {
+        int qx, qy, qz;
+        const int64_t i = ref_index(j, np, &qx, &qy, &qz);
+        double xa[3], ua[3], xb[3], ub[3];
+        view_particle<A64>(a, j, i, xa, ua);
+        view_particle<B64>(b, j, i, xb, ub);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double dx = xa[d] - xb[d];
+            dx -= g * rint(dx / g);
+            const double dv = ua[d] - ub[d];
+            s[0] += dx * dx;
+            s[1] += dv * dv;
+            s[2] += ub[d] * ub[d];
+            s[3] = fmax(s[3], fabs(dx));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int h = CMP_NT / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partial[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // foldParticles (src/power_spectrum/cuda/fold_particles_power.cu:3-27): x -> 2 (x mod ng/2),
 // exact in the split layout.
 __device__ __forceinline__ void fold1(int& c, float& f, int half) {
@@ -1303,6 +1407,27 @@ void launch_fold_split(PosPackH* pos, int64_t n, int ng, hipStream_t s) {
 void launch_force_from_psi(const float* psi, float* out, int ng, hipStream_t s) {
     const int64_t g3 = (int64_t)ng * ng * ng;
     k_force_from_psi<<<grid_blocks(g3, BS), BS, 0, s>>>(psi, out, ng);
+}
+void launch_split_from_f64(const double* p3, const double* v3, PosPackH* pos, float* vel, int64_t n, int np, int ng,
+                           hipStream_t s) {
+    if (!n) return;
+    k_split_from_f64<<<grid_blocks(n, BS), BS, 0, s>>>(p3, v3, (PosPack*)pos, vel, n, np, ng);
+}
+void launch_split_to_f64(const PosPackH* pos, const float* vel, double* p3, double* v3, int64_t n, int np,
+                         hipStream_t s) {
+    if (!n) return;
+    k_split_to_f64<<<grid_blocks(n, BS), BS, 0, s>>>((const PosPack*)pos, vel, p3, v3, n, np);
+}
+int compare_partials() { return CMP_WG; }
+void launch_compare(const PosPackH* pa, const float* va, const double* pa64, const double* va64, const PosPackH* pb,
+                    const float* vb, const double* pb64, const double* vb64, int64_t n, int np, int ng,
+                    double* partial, hipStream_t s) {
+    const ParticleView a{(const PosPack*)pa, va, pa64, va64}, b{(const PosPack*)pb, vb, pb64, vb64};
+    const bool a64 = pa64 != nullptr, b64 = pb64 != nullptr;
+    if (a64 && b64) k_compare<true, true><<<CMP_WG, CMP_NT, 0, s>>>(a, b, n, np, ng, partial);
+    else if (a64) k_compare<true, false><<<CMP_WG, CMP_NT, 0, s>>>(a, b, n, np, ng, partial);
+    else if (b64) k_compare<false, true><<<CMP_WG, CMP_NT, 0, s>>>(a, b, n, np, ng, partial);
+    else k_compare<false, false><<<CMP_WG, CMP_NT, 0, s>>>(a, b, n, np, ng, partial);
 }
 void launch_scan_nonfinite(const float* v, int64_t n, int* flag, hipStream_t s) {
     unsigned nb = grid_blocks(n, BS);

@@ -103,6 +103,9 @@ typedef struct hgpm_ctx hgpm_ctx;
 /* ---------------------------------------------------------------- library ---- */
 const char* hgpm_version(void);
 const char* hgpm_last_error(const hgpm_ctx* ctx);
+/* HIP devices visible to this process (0 when none; never fails for lack of a GPU).
+   No reference counterpart: lets a caller probe for a GPU without creating a context. */
+int hgpm_device_count(int* n);
 
 /* --------------------------------------------------- host-side compat I/O ---- */
 /* read_params (src/simulation/params.cpp:72-251): KEY value file, unknown keys
@@ -163,6 +166,20 @@ int hgpm_get_particles_split(hgpm_ctx* ctx, int32_t* cell, float* offset, float*
 /* FP64 layout (double3 AoS). FP64 contexts only. */
 int hgpm_set_particles_f64(hgpm_ctx* ctx, const double* pos, const double* vel);
 int hgpm_get_particles_f64(hgpm_ctx* ctx, double* pos, double* vel);
+
+/* Particle state from one context to another of the same NP, NG and device, converted
+   between precisions on the device (FP64 -> FP32: c = floor(x), f = float(x - c), an f
+   that rounds to 1 carries; FP32 -> FP64: x = c + f).  No reference counterpart: it
+   hands the FP64 path's initial conditions to the FP32 engine for accuracy runs
+   without a round trip through host memory. */
+int hgpm_copy_particles(hgpm_ctx* dst, hgpm_ctx* src);
+/* Differences of the particle states of two contexts of the same NP, NG and device, in
+   FP64 with a fixed-order reduction: out[0] = sum of squared periodic position
+   differences (grid units, over the 3 components of every particle), out[1] = sum of
+   squared velocity differences, out[2] = sum of b's squared velocities, out[3] = largest
+   position difference of one component.  No reference counterpart (accuracy
+   diagnostics of FP32 runs against FP64 ones). */
+int hgpm_compare_particles(hgpm_ctx* a, hgpm_ctx* b, double* out);
 
 /* serial::GenerateDisplacementIC (src/initial_conditions/initializer.cpp:65-166)
    with the Philox4x32-10 + Box-Muller white noise that replaces cuRAND. ipk is the
@@ -246,8 +263,9 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    = tiles that took the atomic route in the last deposit; "deposit_deferred_full" (get
    only) = those of them that found their slot list or sub-arena full; "deposit_wide"
    (get only) = tiles of the last deposit with a window cell over 32 bits, stored with
-   64-bit cells; "deposit_window_mcells" (get only) = arena cells the last deposit
-   reserved for its windows, in units of 2^20.  Unknown keys return HGPM_ERR_ARG. */
+   64-bit cells; "deposit_arena_mwords" (get only) = 32-bit arena words the last
+   deposit reserved, in units of 2^20 (every window reservation, those of tiles then
+   found full and the doubled ones of the 64-bit pass included).  Unknown keys return HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
 /* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each

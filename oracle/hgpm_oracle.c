@@ -329,10 +329,9 @@ void orc_cic_split(int ng, int64_t n, const int32_t* cell, const float* off, flo
     }
 }
 
-/* CICKernel in FP64 (positions double3). Sequential, deterministic. */
-void orc_cic_f64(int ng, int64_t n, const double* pos, double mass, double* grid) {
-    memset(grid, 0, sizeof(double) * (size_t)ng * ng * ng);
-    for (int64_t p = 0; p < n; p++) {
+/* CIC of particles [p0, p1) into grid (not cleared). */
+static void cic_f64_range(int ng, int64_t p0, int64_t p1, const double* pos, double mass, double* grid) {
+    for (int64_t p = p0; p < p1; p++) {
         double X = pos[3 * p], Y = pos[3 * p + 1], Z = pos[3 * p + 2];
         int i = (int)X, j = (int)Y, k = (int)Z;
         double fx = X - i, fy = Y - j, fz = Z - k;
@@ -346,12 +345,8 @@ void orc_cic_f64(int ng, int64_t n, const double* pos, double mass, double* grid
                 }
     }
 }
-
-/* CICKernel with the reference's float3 positions and FP32 weights; sequential FP32
-   sums in particle order stand in for the FP32 atomics. */
-void orc_cic_ref32(int ng, int64_t n, const float* pos, float mass, float* grid) {
-    memset(grid, 0, sizeof(float) * (size_t)ng * ng * ng);
-    for (int64_t p = 0; p < n; p++) {
+static void cic_ref32_range(int ng, int64_t p0, int64_t p1, const float* pos, float mass, float* grid) {
+    for (int64_t p = p0; p < p1; p++) {
         float X = pos[3 * p], Y = pos[3 * p + 1], Z = pos[3 * p + 2];
         int i = (int)X, j = (int)Y, k = (int)Z;
         float fx = X - (float)i, fy = Y - (float)j, fz = Z - (float)k;
@@ -365,6 +360,54 @@ void orc_cic_ref32(int ng, int64_t n, const float* pos, float mass, float* grid)
                     grid[cidx(ng, (i + x) % ng, (j + y) % ng, (k + z) % ng)] += mul;
                 }
     }
+}
+/* Threads the CIC sums use: T contiguous particle ranges, each summed into a private
+   grid, the grids then added in range order (deterministic for a given T); one range
+   (sequential particle order) when the private grids would exceed 2 GiB. */
+static int cic_ranges(int ng, size_t elem) {
+    int t = 1;
+#ifdef _OPENMP
+    t = omp_get_max_threads();
+#endif
+    double bytes = (double)t * elem * (double)ng * ng * ng;
+    return (t < 1 || bytes > 2147483648.0) ? 1 : t;
+}
+
+/* CICKernel in FP64 (positions double3); sums in a fixed order. */
+void orc_cic_f64(int ng, int64_t n, const double* pos, double mass, double* grid) {
+    const int64_t G = (int64_t)ng * ng * ng;
+    memset(grid, 0, sizeof(double) * (size_t)G);
+    const int T = cic_ranges(ng, sizeof(double));
+    if (T == 1) { cic_f64_range(ng, 0, n, pos, mass, grid); return; }
+    double* priv = (double*)calloc((size_t)T * G, sizeof(double));
+#pragma omp parallel for schedule(static, 1) num_threads(T)
+    for (int t = 0; t < T; t++) cic_f64_range(ng, n * t / T, n * (t + 1) / T, pos, mass, priv + (size_t)t * G);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < G; c++) {
+        double v = 0.0;
+        for (int t = 0; t < T; t++) v += priv[(size_t)t * G + c];
+        grid[c] = v;
+    }
+    free(priv);
+}
+
+/* CICKernel with the reference's float3 positions and FP32 weights, in FP32 sums of a
+   fixed order in place of the reference's FP32 atomics (whose order is undefined). */
+void orc_cic_ref32(int ng, int64_t n, const float* pos, float mass, float* grid) {
+    const int64_t G = (int64_t)ng * ng * ng;
+    memset(grid, 0, sizeof(float) * (size_t)G);
+    const int T = cic_ranges(ng, sizeof(float));
+    if (T == 1) { cic_ref32_range(ng, 0, n, pos, mass, grid); return; }
+    float* priv = (float*)calloc((size_t)T * G, sizeof(float));
+#pragma omp parallel for schedule(static, 1) num_threads(T)
+    for (int t = 0; t < T; t++) cic_ref32_range(ng, n * t / T, n * (t + 1) / T, pos, mass, priv + (size_t)t * G);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < G; c++) {
+        float v = 0.0f;
+        for (int t = 0; t < T; t++) v += priv[(size_t)t * G + c];
+        grid[c] = v;
+    }
+    free(priv);
 }
 
 /* ----------------------------------------------------------------- solve ---- */
@@ -744,34 +787,46 @@ void orc_split_positions(int ng, int64_t n, const double* pos, int32_t* cell, fl
    coeffs: nsteps x 4 doubles (drift0, kick_dt, kick_fscal, drift1) computed by the
    caller's timestepper.  mode ORC_F64: pos/vel double3.  mode ORC_SPLIT: cell/off
    /vel split layout (float32), FP32 forces from an FP64 solve rounded to FP32.
-   Returns elapsed seconds of the step loop. */
-double orc_run_f64(int ng, int np, int nsteps, const double* coeffs, double* pos, double* vel) {
+   Returns elapsed seconds of the step loop; stages (may be NULL) receives the seconds
+   of its stages: [0] drifts, [1] deposit, [2] solve, [3] gather/kick. */
+static double now_s(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+#define STAGE(k, stmt)                          \
+    do {                                        \
+        double t_ = now_s();                    \
+        stmt;                                   \
+        st[k] += now_s() - t_;                  \
+    } while (0)
+
+double orc_run_f64(int ng, int np, int nsteps, const double* coeffs, double* pos, double* vel, double* stages) {
     int64_t N = (int64_t)np * np * np, G = (int64_t)ng * ng * ng;
     double mass = pow((double)ng / (double)np, 3.0);
     double* rho = (double*)malloc(sizeof(double) * G);
     double* force = (double*)malloc(sizeof(double) * 3 * G);
-#ifdef _OPENMP
-    double t0 = omp_get_wtime();
-#endif
+    double st[4] = {0, 0, 0, 0};
+    double t0 = now_s();
     for (int s = 0; s < nsteps; s++) {
         const double* c = coeffs + 4 * s;
-        orc_drift_f64(ng, N, pos, vel, c[0]);
-        orc_cic_f64(ng, N, pos, mass, rho);
-        orc_solve_gradient(ng, ORC_F64, rho, force);
-        orc_kick_f64(ng, N, pos, vel, force, c[1], c[2]);
-        orc_drift_f64(ng, N, pos, vel, c[3]);
+        STAGE(0, orc_drift_f64(ng, N, pos, vel, c[0]));
+        STAGE(1, orc_cic_f64(ng, N, pos, mass, rho));
+        STAGE(2, orc_solve_gradient(ng, ORC_F64, rho, force));
+        STAGE(3, orc_kick_f64(ng, N, pos, vel, force, c[1], c[2]));
+        STAGE(0, orc_drift_f64(ng, N, pos, vel, c[3]));
     }
-    double el = 0.0;
-#ifdef _OPENMP
-    el = omp_get_wtime() - t0;
-#endif
+    double el = now_s() - t0;
+    if (stages) memcpy(stages, st, sizeof(st));
     free(rho);
     free(force);
     return el;
 }
 
 double orc_run_split(int ng, int np, int nsteps, const double* coeffs, int32_t* cell, float* off,
-                     float* vel) {
+                     float* vel, double* stages) {
     int64_t N = (int64_t)np * np * np, G = (int64_t)ng * ng * ng;
     float gp = (float)ng / (float)np;
     float mass = gp * gp * gp;
@@ -779,28 +834,27 @@ double orc_run_split(int ng, int np, int nsteps, const double* coeffs, int32_t* 
     double* rho = (double*)malloc(sizeof(double) * G);
     double* force = (double*)malloc(sizeof(double) * 3 * G);
     float* ff = (float*)malloc(sizeof(float) * 3 * G);
-#ifdef _OPENMP
-    double t0 = omp_get_wtime();
-#endif
+    double st[4] = {0, 0, 0, 0};
+    double t0 = now_s();
     for (int s = 0; s < nsteps; s++) {
         const double* c = coeffs + 4 * s;
-        orc_drift_split(ng, N, cell, off, vel, (float)c[0]);
-        orc_cic_split(ng, N, cell, off, mass, q);
-        for (int64_t i = 0; i < G; i++) rho[i] = (double)q[i] * (1.0 / 16777216.0);
-        orc_solve_gradient(ng, ORC_F64, rho, force);
-        for (int64_t i = 0; i < 3 * G; i++) ff[i] = (float)force[i];
-        orc_kick_split(ng, N, cell, off, vel, ff, c[1], c[2]);
-        orc_drift_split(ng, N, cell, off, vel, (float)c[3]);
+        STAGE(0, orc_drift_split(ng, N, cell, off, vel, (float)c[0]));
+        STAGE(1, orc_cic_split(ng, N, cell, off, mass, q));
+        STAGE(2, {
+            for (int64_t i = 0; i < G; i++) rho[i] = (double)q[i] * (1.0 / 16777216.0);
+            orc_solve_gradient(ng, ORC_F64, rho, force);
+            for (int64_t i = 0; i < 3 * G; i++) ff[i] = (float)force[i];
+        });
+        STAGE(3, orc_kick_split(ng, N, cell, off, vel, ff, c[1], c[2]));
+        STAGE(0, orc_drift_split(ng, N, cell, off, vel, (float)c[3]));
     }
-    double el = 0.0;
-#ifdef _OPENMP
-    el = omp_get_wtime() - t0;
-#endif
+    double el = now_s() - t0;
+    if (stages) memcpy(stages, st, sizeof(st));
     free(q); free(rho); free(force); free(ff);
     return el;
 }
 
-double orc_run_ref32(int ng, int np, int nsteps, const double* coeffs, float* pos, float* vel) {
+double orc_run_ref32(int ng, int np, int nsteps, const double* coeffs, float* pos, float* vel, double* stages) {
     int64_t N = (int64_t)np * np * np, G = (int64_t)ng * ng * ng;
     float gp = (float)ng / (float)np;
     float mass = gp * gp * gp;
@@ -808,23 +862,22 @@ double orc_run_ref32(int ng, int np, int nsteps, const double* coeffs, float* po
     double* rho = (double*)malloc(sizeof(double) * G);
     double* force = (double*)malloc(sizeof(double) * 3 * G);
     float* ff = (float*)malloc(sizeof(float) * 3 * G);
-#ifdef _OPENMP
-    double t0 = omp_get_wtime();
-#endif
+    double st[4] = {0, 0, 0, 0};
+    double t0 = now_s();
     for (int s = 0; s < nsteps; s++) {
         const double* c = coeffs + 4 * s;
-        orc_drift_ref32(ng, N, pos, vel, (float)c[0]);
-        orc_cic_ref32(ng, N, pos, mass, rf);
-        for (int64_t i = 0; i < G; i++) rho[i] = rf[i];
-        orc_solve_gradient(ng, ORC_REF32, rho, force);
-        for (int64_t i = 0; i < 3 * G; i++) ff[i] = (float)force[i];
-        orc_kick_ref32(ng, N, pos, vel, ff, c[1], c[2]);
-        orc_drift_ref32(ng, N, pos, vel, (float)c[3]);
+        STAGE(0, orc_drift_ref32(ng, N, pos, vel, (float)c[0]));
+        STAGE(1, orc_cic_ref32(ng, N, pos, mass, rf));
+        STAGE(2, {
+            for (int64_t i = 0; i < G; i++) rho[i] = rf[i];
+            orc_solve_gradient(ng, ORC_REF32, rho, force);
+            for (int64_t i = 0; i < 3 * G; i++) ff[i] = (float)force[i];
+        });
+        STAGE(3, orc_kick_ref32(ng, N, pos, vel, ff, c[1], c[2]));
+        STAGE(0, orc_drift_ref32(ng, N, pos, vel, (float)c[3]));
     }
-    double el = 0.0;
-#ifdef _OPENMP
-    el = omp_get_wtime() - t0;
-#endif
+    double el = now_s() - t0;
+    if (stages) memcpy(stages, st, sizeof(st));
     free(rf); free(rho); free(force); free(ff);
     return el;
 }

@@ -56,9 +56,9 @@ def L():
             "orc_pk_interp": (cd, [ci, ci, ci, vp, cd, cd, ci]),
             "orc_ic_f64": (None, [ci, ci, cd, ctypes.c_uint32, vp, ci, cd, cd, cd, cd, cd, cd, vp, vp]),
             "orc_split_positions": (None, [ci, i64, vp, vp, vp]),
-            "orc_run_f64": (cd, [ci, ci, ci, vp, vp, vp]),
-            "orc_run_split": (cd, [ci, ci, ci, vp, vp, vp, vp]),
-            "orc_run_ref32": (cd, [ci, ci, ci, vp, vp, vp]),
+            "orc_run_f64": (cd, [ci, ci, ci, vp, vp, vp, vp]),
+            "orc_run_split": (cd, [ci, ci, ci, vp, vp, vp, vp, vp]),
+            "orc_run_ref32": (cd, [ci, ci, ci, vp, vp, vp, vp]),
             "orc_num_threads": (ci, []),
             "orc_set_num_threads": (None, [ci]),
         }
@@ -141,6 +141,21 @@ def kick_f64(ng, pos, vel, force, deltaT, fscal):
     return v
 
 
+def kick_ref32(ng, pos, vel, force, deltaT, fscal):
+    pos = np.ascontiguousarray(pos, np.float32)
+    v = np.ascontiguousarray(vel, np.float32).copy()
+    force = np.ascontiguousarray(force, np.float32)
+    L().orc_kick_ref32(ng, pos.size // 3, _p(pos), _p(v), _p(force), float(deltaT), float(fscal))
+    return v
+
+
+def drift_ref32(ng, pos, vel, pf):
+    p = np.ascontiguousarray(pos, np.float32).copy()
+    v = np.ascontiguousarray(vel, np.float32)
+    L().orc_drift_ref32(ng, p.size // 3, _p(p), _p(v), float(pf))
+    return p
+
+
 def drift_split(ng, cell, off, vel, pf):
     c = np.ascontiguousarray(cell, np.int32).copy()
     o = np.ascontiguousarray(off, np.float32).copy()
@@ -203,28 +218,41 @@ def split_positions(ng, pos):
     return cell, off
 
 
-def run_f64(ng, np_, coeffs, pos, vel):
+STAGES = ("drift", "deposit", "solve", "gather_kick")
+
+
+def run_f64(ng, np_, coeffs, pos, vel, stages=None):
+    """stages: a dict that receives the seconds of each step stage (STAGES)."""
     c = np.ascontiguousarray(coeffs, np.float64)
     p = np.ascontiguousarray(pos, np.float64).copy()
     v = np.ascontiguousarray(vel, np.float64).copy()
-    t = L().orc_run_f64(ng, np_, c.shape[0], _p(c), _p(p), _p(v))
+    st = np.zeros(4)
+    t = L().orc_run_f64(ng, np_, c.shape[0], _p(c), _p(p), _p(v), _p(st))
+    if stages is not None:
+        stages.update(zip(STAGES, st.tolist()))
     return p, v, t
 
 
-def run_split(ng, np_, coeffs, cell, off, vel):
+def run_split(ng, np_, coeffs, cell, off, vel, stages=None):
     c = np.ascontiguousarray(coeffs, np.float64)
     ce = np.ascontiguousarray(cell, np.int32).copy()
     o = np.ascontiguousarray(off, np.float32).copy()
     v = np.ascontiguousarray(vel, np.float32).copy()
-    t = L().orc_run_split(ng, np_, c.shape[0], _p(c), _p(ce), _p(o), _p(v))
+    st = np.zeros(4)
+    t = L().orc_run_split(ng, np_, c.shape[0], _p(c), _p(ce), _p(o), _p(v), _p(st))
+    if stages is not None:
+        stages.update(zip(STAGES, st.tolist()))
     return ce, o, v, t
 
 
-def run_ref32(ng, np_, coeffs, pos, vel):
+def run_ref32(ng, np_, coeffs, pos, vel, stages=None):
     c = np.ascontiguousarray(coeffs, np.float64)
     p = np.ascontiguousarray(pos, np.float32).copy()
     v = np.ascontiguousarray(vel, np.float32).copy()
-    t = L().orc_run_ref32(ng, np_, c.shape[0], _p(c), _p(p), _p(v))
+    st = np.zeros(4)
+    t = L().orc_run_ref32(ng, np_, c.shape[0], _p(c), _p(p), _p(v), _p(st))
+    if stages is not None:
+        stages.update(zip(STAGES, st.tolist()))
     return p, v, t
 
 
