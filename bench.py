@@ -12,11 +12,17 @@ Also reported on the same line:
   roofline     : the dominant kernel (largest share of the timed region, timed with HIP
                  events on the engine stream) against MI355X HBM bandwidth, from its
                  algorithmic bytes per launch;
-  fp64_plain   : the plain FP64 GPU path (reference-structured kernels in FP64) on the
-                 same workload, and the FP32/FP64 speed-up;
-  cpu_baseline : the reference algorithm restated on the host CPU (the oracle in the
-                 reference's FP32 rounding) on 192^3 particles x 50 steps at the same
-                 resolution, threads stated.
+  accuracy     : the whole run on the FP32 engine and on the plain FP64 GPU path
+                 (reference-structured kernels in FP64) from the same initial
+                 conditions: max relative P(k) error for k < k_Nyq/2, RMS displacement
+                 (grid units) and velocity errors, the density bit-exact across deposits;
+  fp64_plain   : the FP64 path's rate over that run, and the FP32/FP64 speed-up;
+  cpu_baseline : the reference algorithm restated on the host CPU (the oracle) on the
+                 64^3-particle, 120-step workload, in the reference's FP32 rounding and
+                 in FP64, on all host threads and on one, per-stage seconds;
+  ensemble     : 96 boxes (32 each of 64^3, 128^3 and 192^3 particles, per-box seed and
+                 sigma_8) stepped 40 steps z = 30 -> 0 as a batched ensemble (HIP graph
+                 runs) and one box at a time, and the speed-up.
 """
 import argparse
 import json
@@ -132,7 +138,7 @@ def run_full(hg, p, per_kernel=True):
     sched = hg.schedule(p, n, merge=True)
     t = []
     deferred = []
-    windows = []  # arena cells reserved per grid cell, per step
+    arena = []  # 32-bit arena words reserved per grid cell, per step
     regions = []
     with hg.MemoryManager(p) as m:
         m.autotune(3)
@@ -148,7 +154,7 @@ def run_full(hg, p, per_kernel=True):
             regions.append(m.timings() if per_kernel else {})
             deferred.append([m.get_tunable(k) for k in
                              ("deposit_deferred", "deposit_wide", "deposit_deferred_full")])
-            windows.append(m.get_tunable("deposit_window_mcells") / (p.ng ** 3 / 2 ** 20))
+            arena.append(m.get_tunable("deposit_arena_mwords") / (p.ng ** 3 / 2 ** 20))
     k = max(1, n // 10)
     Np = p.np ** 3
 
@@ -162,7 +168,9 @@ def run_full(hg, p, per_kernel=True):
             # per step: tiles deferred, tiles with 64-bit windows, tiles deferred for a full
             # slot list or sub-arena
             "deferred_tiles": deferred,
-            "window_cells_per_grid_cell": [round(w, 3) for w in windows],
+            # every window reservation (those of tiles then found full and the doubled ones
+            # of the 64-bit pass included), in 32-bit words per grid cell
+            "arena_words_per_grid_cell": [round(w, 3) for w in arena],
             "per_kernel": per_kernel,
             "kernel_ms_first_10pct": region_ms(regions[:k]), "kernel_ms_last_10pct": region_ms(regions[-k:]),
             "ms_per_step": [round(1e3 * x, 3) for x in t]}
@@ -190,16 +198,28 @@ def copy_ceiling(gib=8, reps=3):
             "note": "torch add(a, 0, out=b), best of %d" % reps}
 
 
-def cpu_baseline(hg, threads, ng=192):
-    """Reference algorithm on the host: the oracle in the reference's FP32 rounding
-    (float3 positions, FP32 CIC, reference calcGreens), on a bounded sample of the same
-    workload: ng^3 particles at the same resolution (box 2000*ng/1024 Mpc/h), z = 50 -> 0
-    in 50 steps."""
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hg, threads):
+    """The reference algorithm on the host: the oracle's restatement of the per-step PM
+    loop on the CPU-runnable workload, 64^3 particles on a 64^3 grid, box 128 Mpc/h,
+    Zel'dovich initial conditions at z = 200 (seed 21082001, the reference's own
+    test_run_params), 120 a-uniform steps to z = 0 (with 60 steps the reference's
+    initial half step back makes a < 0).  Two precisions: the reference's FP32 rounding
+    (float3 positions, FP32 CIC sums in a fixed order, the reference's calcGreens
+    rounding; `value`) and FP64.  On `threads` OpenMP threads, and once on 1 thread."""
     from oracle import oracle as orc
-    orc.set_threads(threads)
-    n = 50
-    rl = 2000.0 * ng / 1024
-    p = hg.make_params(ng, ng, rl, 50.0, 0.0, n, seed=2)
+    ng, n = 64, 120
+    p = hg.make_params(ng, ng, 128.0, 200.0, 0.0, n, seed=21082001)
     ts = hg.Timestepper(p)
     ts.setInitialZ(p.z_ini)
     ts.reverseHalfStep()
@@ -207,14 +227,31 @@ def cpu_baseline(hg, threads, ng=192):
     a = 1 / (1 + p.z_ini)
     D, _ = hg.growth(p, a)
     _, dD = hg.growth(p, a - ts.deltaT / 2)
-    pos, vel = orc.ic_f64(ng, ng, rl, 2, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
+    pos, vel = orc.ic_f64(ng, ng, p.rl, p.seed, vals, hdr[3], D, dD, p.z_ini, ts.fscal, ts.deltaT)
     co = hg.schedule_array(p, n, merge=False)
-    t0 = time.perf_counter()
-    _, _, t_loop = orc.run_ref32(ng, ng, co, pos.astype(np.float32), vel.astype(np.float32))
-    el = time.perf_counter() - t0
-    return {"value": ng ** 3 * n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{ng}^3 particles, {ng}^3 grid, box {rl:g} Mpc/h, z=50->0, {n} steps (reference "
-                      f"FP32 rounding, FP64 FFT), {el:.1f} s"}
+    Np = ng ** 3
+    runs = {}
+    for label, fn, th in [("ref32", orc.run_ref32, threads), ("f64", orc.run_f64, threads),
+                          ("ref32_1thread", orc.run_ref32, 1)]:
+        orc.set_threads(th)
+        st = {}
+        t0 = time.perf_counter()
+        if label.startswith("ref32"):
+            fn(ng, ng, co, pos.astype(np.float32), vel.astype(np.float32), st)
+        else:
+            fn(ng, ng, co, pos, vel, st)
+        el = time.perf_counter() - t0
+        runs[label] = {"threads": th, "seconds": el, "particle_steps_per_s": Np * n / el,
+                       "stage_seconds": {k: round(v, 3) for k, v in st.items()}}
+    orc.set_threads(threads)
+    return {"value": runs["ref32"]["particle_steps_per_s"], "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "model": cpu_model(), "threads": threads,
+            "value_1thread": runs["ref32_1thread"]["particle_steps_per_s"],
+            "value_f64": runs["f64"]["particle_steps_per_s"],
+            "per_stage": runs["ref32"]["stage_seconds"], "runs": runs,
+            "sample": f"64^3 particles, 64^3 grid, box 128 Mpc/h, z=200->0 in {n} steps, seed 21082001: the "
+                      f"oracle in the reference's FP32 rounding (value) and in FP64, {threads} threads, "
+                      f"and the FP32 rounding on 1 thread"}
 
 
 def ensemble_boxes(hg, per_class=32, sizes=(64, 128, 192), z_ini=30.0, n_steps=40):
@@ -305,7 +342,7 @@ def ic_args(hg, p):
     return hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT
 
 
-def run_ensemble(hg, steps, warmup, per_class):
+def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
     """Batched ensemble, its steps captured into one HIP graph, vs the same boxes
     launched one at a time (one context per box, each box's step completed before the
     next box's starts).  The batched run with eager launches is reported beside it."""
@@ -354,9 +391,10 @@ def run_ensemble(hg, steps, warmup, per_class):
                 m.synchronize()
             t_single += time.perf_counter() - t0
     # each size class batched on its own (no other class sharing the GPU)
-    progress("each size class alone")
     classes = []
-    for n in sorted({p.np for p in ps}):
+    if classes_alone:
+        progress("each size class alone")
+    for n in sorted({p.np for p in ps}) if classes_alone else []:
         idx = [b for b, p in enumerate(ps) if p.np == n]
         with hg.Ensemble([ps[b] for b in idx]) as e:
             e.autotune(2)
@@ -394,9 +432,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ng", type=int, default=1024)
     ap.add_argument("--rl", type=float, default=2000.0)
-    ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the whole z_ini -> z_fin run timed step by step")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -416,6 +454,10 @@ def main():
     ap.add_argument("--only-copy", action="store_true",
                     help="run only the --copy-calibration copy (a separate profiler pass)")
     args = ap.parse_args()
+    if args.copy_calibration and not args.only_copy:
+        # torch's HIP runtime and libhgpm's must not share a process: the calibration
+        # copy runs alone, as its own profiler pass
+        ap.error("--copy-calibration runs only with --only-copy (a process of its own)")
     if args.only_ceiling:
         print(json.dumps(copy_ceiling()))
         return
@@ -519,29 +561,38 @@ def main():
         out["hbm_streaming_ceiling"] = (json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0
                                         else {"error": r.stderr.strip()[-300:]})
     if not args.no_fp64:
-        p64 = hg.make_params(ng, ng, args.rl, 50.0, 0.0, n_run, seed=2, pk_bins=221, precision=hg.FP64)
+        # the accuracy clause of the metric, on the same workload: the whole run on the
+        # FP32 engine and on the FP64 plain path from the FP64 path's initial conditions
+        # (hacc_gpm_amd.accuracy); the FP64 path's rate over that run is the plain GPU
+        # baseline of the speed-up
         try:
-            progress("FP64 plain path")
-            el64, _, _, b64, _ = run_engine(hg, p64, args.fp64_steps, 1, timing=False)
-            v64 = Np * args.fp64_steps / el64
-            out["fp64_plain"] = {"value": v64, "ms_per_step": el64 * 1e3 / args.fp64_steps,
-                                 "steps": args.fp64_steps, "device_GB": b64 / 1e9}
+            from hacc_gpm_amd import accuracy
+            progress(f"accuracy: whole run on the FP32 engine and the FP64 plain path ({n_run} steps)")
+            acc = accuracy.compare(ng, ng, args.rl, 50.0, n_run, 2, pk_bins=221, quiet=True)
+            out["accuracy"] = {k: acc[k] for k in ("workload", "pk_max_rel_err", "pk_bins_compared",
+                                                   "pk_counts_equal", "rms_disp_cells", "max_disp_cells",
+                                                   "vel_rel_rms", "density_bit_exact", "ms_per_step_fp32",
+                                                   "ms_per_step_fp64")}
+            out["accuracy"]["pk_err_bound"] = 1e-3
+            v64 = Np * 1e3 / acc["ms_per_step_fp64"]
+            out["fp64_plain"] = {"value": v64, "ms_per_step": acc["ms_per_step_fp64"], "steps": n_run}
             out["speedup_vs_fp64"] = value / v64
         except Exception as e:  # report, do not hide
-            out["fp64_plain"] = {"error": str(e)}
+            out["accuracy"] = {"error": str(e)}
+    if not args.no_ensemble:
+        # the batched-ensemble workload (96 boxes of 64^3, 128^3 and 192^3, 40 steps
+        # z = 30 -> 0) against the same boxes one at a time
+        progress("ensemble: 3 x 32 boxes batched (HIP graph), then one at a time")
+        try:
+            ens = run_ensemble(hg, max(1, 40 - 3), 3, 32, classes_alone=False)
+            out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "one_at_a_time", "batched_eager",
+                                                   "speedup", "batches", "config")}
+        except Exception as e:  # report, do not hide
+            out["ensemble"] = {"error": str(e)}
     if not args.no_cpu:
         th = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         progress(f"CPU baseline on {th} threads")
         out["cpu_baseline"] = cpu_baseline(hg, th)
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration << 20
-        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = n
     print(json.dumps(out))
 
 

@@ -160,16 +160,33 @@ static void fft1d_rec(const double* in, int stride, double* out, int n, const do
     double* t = tmp;
     double* heap = NULL;
     if (p > 64) { heap = (double*)malloc(sizeof(double) * 2 * p); t = heap; }
+    if (p == 2) {
+        /* radix 2: X[k] = E[k] + w^k O[k], X[k + m] = E[k] - w^k O[k], w^k = W[k wstep] */
+        for (int k = 0; k < m; k++) {
+            const double er = out[2 * k], ei = out[2 * k + 1];
+            const double or_ = out[2 * (m + k)], oi = out[2 * (m + k) + 1];
+            const double wr = W[2 * (int64_t)k * wstep], wi = W[2 * (int64_t)k * wstep + 1];
+            const double tr = or_ * wr - oi * wi, ti = or_ * wi + oi * wr;
+            out[2 * k] = er + tr; out[2 * k + 1] = ei + ti;
+            out[2 * (m + k)] = er - tr; out[2 * (m + k) + 1] = ei - ti;
+        }
+        if (heap) free(heap);
+        return;
+    }
     for (int k = 0; k < m; k++) {
         for (int r = 0; r < p; r++) { t[2 * r] = out[2 * (r * m + k)]; t[2 * r + 1] = out[2 * (r * m + k) + 1]; }
         for (int q = 0; q < p; q++) {
             double sr = 0, si = 0;
             int kk = k + q * m;
+            /* twiddle index r kk wstep mod Nfull, stepped by kk wstep mod Nfull */
+            const int64_t de = ((int64_t)kk * wstep) % Nfull;
+            int64_t e = 0;
             for (int r = 0; r < p; r++) {
-                int64_t e = ((int64_t)r * kk * wstep) % Nfull;
                 double wr = W[2 * e], wi = W[2 * e + 1];
                 sr += t[2 * r] * wr - t[2 * r + 1] * wi;
                 si += t[2 * r] * wi + t[2 * r + 1] * wr;
+                e += de;
+                if (e >= Nfull) e -= Nfull;
             }
             out[2 * kk] = sr; out[2 * kk + 1] = si;
         }

@@ -143,6 +143,56 @@ def test_deposit_mass_ratio(hg, orc):
         assert np.array_equal(m.get_density(), orc.cic_split(64, cell, off, 8.0))
 
 
+@pytest.mark.parametrize("ng,np_,sigma", [(192, 32, 1.5), (192, 32, 0.2), (256, 32, 1.5)])
+def test_deposit_heavy_particles(hg, orc, ng, np_, sigma):
+    """Particles of mass (ng/np)^3 >= 128 (216 at 192/32, 512 at 256/32) take the
+    deposit kernels built for heavy particles (64-bit weights per particle).  A 16^3
+    Lagrangian tile spans 96-128 cells per side, so its window is staged in many slabs;
+    at mass 512 one particle's weight can exceed 2^32 - 1 units, and with sigma 0.2 the
+    cells of many tiles do: both the 32-bit windows and the 64-bit pass run, then the
+    atomic route under a short arena.  All bit-exact against the oracle, and a step over
+    these particles matches the separate calls bit for bit."""
+    case(f"deposit heavy ng={ng} np={np_} sigma={sigma}")
+    rng = np.random.default_rng(ng + np_ + int(10 * sigma))
+    mass = (ng / np_) ** 3
+    p = hg.make_params(ng, np_, 2.0 * ng, 50.0, 0.0, 100)
+    with hg.MemoryManager(p) as m, hg.MemoryManager(p) as m2:
+        cell, off = lattice_split(rng, np_, ng, sigma, outliers=2)
+        if sigma < 1:  # a clump: 4096 particles piled into a few cells
+            x = (np.array([50.0, 70.0, 90.0]) + rng.normal(0, 0.4, (4096, 3))) % ng
+            cell[:4096] = np.floor(x).astype(np.int32) % ng
+            off[:4096] = (x - np.floor(x)).astype(np.float32)
+            off[off >= 1.0] = 0.0
+        vel = rng.normal(0, 0.5, (m.N, 3)).astype(np.float32)
+        ref = orc.cic_split(ng, cell, off, mass)
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        wide, deferred = m.get_tunable("deposit_wide"), m.get_tunable("deposit_deferred")
+        case(f"  routes: wide {wide} deferred {deferred}")
+        if ng == 192 and sigma > 1:
+            assert deferred == 0  # 7^3 owner blocks per tile at most: every tile on the arena
+        else:
+            # at 256/32 a tile's window spans up to 9^3 owner blocks (over the 512-chunk
+            # limit: the atomic route); heavy cells take the 64-bit pass
+            assert wide + deferred > 0
+        m.set_tunable("deposit_arena", 0)  # every tile on the atomic route
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_deferred") == m.N // 4096
+        m.set_tunable("deposit_arena", -1)
+        m2.set_particles_split(cell, off, vel)
+        for co in hg.schedule(p, 3, merge=True):
+            m.step(co)
+            if not co.skip_drift0:
+                m2.update_positions(co.drift0)
+            m2.cic()
+            m2.solve_gradient()
+            m2.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
+            for a, b in zip(m.get_particles_split(), m2.get_particles_split()):
+                assert np.array_equal(a, b)
+
+
 # ------------------------------------------------------------------- FFT ----
 @pytest.mark.parametrize("ng", [32, 64, 96, 128, 192])
 def test_fft_fp32(hg, ng):
@@ -233,6 +283,42 @@ def test_solve_fp32_vs_fp64_path_large(hg, ng):
         m64.solve_gradient()
         F64 = m64.get_force()
     assert rel_l2(F.astype(np.float64), F64) < 2e-5
+
+
+@pytest.mark.parametrize("ng", [512, 768, 1024])
+def test_solve_single_mode_closed_form_large(hg, ng):
+    """The FP32 half-spectrum solve at the large plans against the closed form, with no
+    other code path in between: rho = 1 + A cos(k.x) gives, under the reference's
+    Green's function (calcGreens.hpp:11-26) and central-difference gradient
+    (kspace_solver.cu:55-90), F_d = A sin(k_d) sin(k.x) / (2 sum_d (cos k_d - 1)).
+    The density goes in as int64 2^-24 units (rounding error 2^-25, relative 1e-7 of A).
+    Relative RMS error over 2^21 sampled cells < 2e-5."""
+    case(f"solve single mode closed form ng={ng}")
+    A = 0.3
+    m_ = np.array([ng // 8 + 1, 5, ng // 3])
+    j = np.arange(ng, dtype=np.float64)
+    table = np.rint((1.0 + A * np.cos(2 * np.pi * j / ng)) * 2.0 ** 24).astype(np.int64)
+    ax = np.arange(ng, dtype=np.int64)
+    idx = ((m_[0] * ax[:, None, None] + m_[1] * ax[None, :, None]) % ng + m_[2] * ax[None, None, :]) % ng
+    rho = table[idx]
+    del idx
+    with mk(hg, ng, 32) as m:
+        m.set_density(rho)
+        del rho
+        m.solve_gradient()
+        F = m.get_force()
+    rng = np.random.default_rng(ng)
+    s = rng.integers(0, ng ** 3, 1 << 21)
+    x, y, z = s // (ng * ng), (s // ng) % ng, s % ng
+    k = 2 * np.pi * m_ / ng
+    phase = 2 * np.pi * ((m_[0] * x + m_[1] * y + m_[2] * z) % ng) / ng
+    lam = 2 * np.sum(np.cos(k) - 1)
+    err2 = ref2 = 0.0
+    for d in range(3):
+        expect = A * np.sin(k[d]) * np.sin(phase) / lam
+        err2 += float(np.sum((F[s, d].astype(np.float64) - expect) ** 2))
+        ref2 += float(np.sum(expect ** 2))
+    assert np.sqrt(err2 / ref2) < 2e-5, np.sqrt(err2 / ref2)
 
 
 @pytest.mark.parametrize("ng", [32, 96, 128])
