@@ -58,6 +58,8 @@ class Params(ctypes.Structure):
         ("dump_init", ctypes.c_int), ("dump_final", ctypes.c_int),
         ("ol", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("analysis", ctypes.c_ubyte * MAX_STEPS), ("analysis_dir", ctypes.c_char * 300),
+        ("analysis_py", ctypes.c_char * 300), ("do_analysis", ctypes.c_int),
     ]
 
 
@@ -110,6 +112,7 @@ def lib():
             "hgpm_set_particles_f64": (c_int, [vp, vp, vp]),
             "hgpm_get_particles_f64": (c_int, [vp, vp, vp]),
             "hgpm_copy_particles": (c_int, [vp, vp]),
+            "hgpm_analysis_particles": (c_int, [vp, vp]),
             "hgpm_compare_particles": (c_int, [vp, vp, vp]),
             "hgpm_generate_ic": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double, c_double]),
             "hgpm_update_positions": (c_int, [vp, c_double]),
@@ -358,6 +361,14 @@ class MemoryManager:
         vel = np.zeros((self.N, 3), np.float64)
         self._c(lib().hgpm_get_particles_f64(self.h, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def analysis_particles(self):
+        """The array serial::PyAnalysis hands to the analysis module
+        (src/analysis/cuda/py_analysis.cu:3-26): (np^3, 7) float32 rows of
+        (index, x, y, z, vx, vy, vz) in the reference's particle order."""
+        out = np.zeros((self.N, 7), np.float32)
+        self._c(lib().hgpm_analysis_particles(self.h, _ptr(out)))
+        return out
 
     def copy_particles_from(self, other):
         """Particle state of another context (same NP, NG), converted on the device."""
@@ -616,6 +627,18 @@ def GetPowerSpectrum(params, mem, fname):
 def writeOutput(params, mem, fname):
     """serial::writeOutput (io.cu:8-44)."""
     mem.write_output(fname)
+
+
+def PyAnalysis(params, mem, ts, analysis, step):
+    """serial::PyAnalysis (src/analysis/cuda/py_analysis.cu:3-26): with DO_ANALYSIS set
+    and `step` listed in ANALYSIS_STEPS, calls analysis(particles, n, 7, step, a, z) with
+    the (n, 7) float32 array of mem.analysis_particles(); `analysis` stands for the
+    analysisStep of the reference's ANALYSIS_PY module.  Returns whether it was called."""
+    if not (params.do_analysis and params.analysis[step]):
+        return False
+    parts = mem.analysis_particles()
+    analysis(parts, parts.shape[0], 7, step, ts.aa, ts.z)
+    return True
 
 
 def GenerateDisplacementIC(mem, params, ts, ipk_header, ipk_values):

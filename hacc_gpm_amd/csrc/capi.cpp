@@ -647,6 +647,27 @@ int hgpm_get_particles(hgpm_ctx* c, float* pos, float* vel) {
     return HGPM_OK;
 }
 
+// serial::PyAnalysis (src/analysis/cuda/py_analysis.cu:3-26): the snapshot's float3
+// positions and velocities interleaved as (index, x, y, z, vx, vy, vz) per particle.
+int hgpm_analysis_particles(hgpm_ctx* c, float* out) {
+    if (!c || !out) return fail(c, HGPM_ERR_ARG, "hgpm_analysis_particles: null argument");
+    const size_t n = (size_t)c->N;
+    std::vector<float> pos(3 * n), vel(3 * n);
+    int rc = hgpm_get_particles(c, pos.data(), vel.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < n; i++) {
+        float* o = out + 7 * i;
+        o[0] = (float)i;
+        o[1] = pos[3 * i];
+        o[2] = pos[3 * i + 1];
+        o[3] = pos[3 * i + 2];
+        o[4] = vel[3 * i];
+        o[5] = vel[3 * i + 1];
+        o[6] = vel[3 * i + 2];
+    }
+    return HGPM_OK;
+}
+
 int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off, const float* vel) {
     if (!c || !cell || !off || !vel) return fail(c, HGPM_ERR_ARG, "hgpm_set_particles_split: null argument");
     if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "split layout needs an FP32 context");

@@ -5,11 +5,21 @@
 //         advanceHalfStep
 //   loop: drift 1/2, CIC, SolveGradient, a += dt/2, kick, a += dt/2, drift 1/2,
 //         <prefix>.pk.<step> / <prefix>.particles.<step> on the listed steps
+//         and, with DO_ANALYSIS, the analysis hook on the ANALYSIS_STEPS (below)
 //   end : <prefix>.pk.fin, optional <prefix>.particles.fin, timers
+// Analysis hook (serial::PyAnalysis, src/analysis/cuda/py_analysis.cu:3-26,
+// driver_pm.cpp:84-91,153): the reference hands 7 floats per particle (index, x, y, z,
+// vx, vy, vz) and (step, a, z) to a function of the ANALYSIS_PY module through an
+// embedded interpreter.  Here the same array is written raw (float32, particle-major) to
+// <ANALYSIS_DIR>/<prefix name>.analysis.<step>, and ANALYSIS_PY, when given, runs as a
+// child process: python3 ANALYSIS_PY <file> <particles> 7 <step> <a> <z> <ANALYSIS_DIR>.
 // The two half drifts between steps without output run as one drift (fused into the
 // kick kernel).  A non-zero status from the engine prints its message and exits 1.
 // Only the serial path exists: a `+nN` argument with N > 1 is refused.
 #include "hgpm.h"
+
+#include <spawn.h>
+#include <sys/wait.h>
 
 #include <chrono>
 #include <cstdio>
@@ -21,6 +31,43 @@
 static void check(int rc, hgpm_ctx* c, const char* what) {
     if (rc != HGPM_OK) {
         fprintf(stderr, "haccgpm: %s failed (status %d): %s\n", what, rc, hgpm_last_error(c));
+        exit(1);
+    }
+}
+
+extern char** environ;
+
+// The analysis step of the hook: the snapshot file, then the analysis script on it.
+static void analysis_step(hgpm_ctx* ctx, const hgpm_params& p, int step, const hgpm_timestepper& ts) {
+    const size_t n = (size_t)p.np * p.np * p.np;
+    std::vector<float> buf(7 * n);
+    check(hgpm_analysis_particles(ctx, buf.data()), ctx, "PyAnalysis");
+    std::string dir = p.analysis_dir[0] ? p.analysis_dir : ".";
+    std::string name = p.prefix;
+    const size_t sl = name.rfind('/');
+    if (sl != std::string::npos) name = name.substr(sl + 1);
+    const std::string fn = dir + "/" + name + ".analysis." + std::to_string(step);
+    FILE* f = fopen(fn.c_str(), "wb");
+    if (!f || fwrite(buf.data(), sizeof(float), buf.size(), f) != buf.size()) {
+        fprintf(stderr, "haccgpm: cannot write %s\n", fn.c_str());
+        exit(1);
+    }
+    fclose(f);
+    printf("Doing Python Analysis Step %d: %s\n", step, fn.c_str());
+    if (!p.analysis_py[0]) return;
+    char a[64], z[64], ns[32], st[32];
+    snprintf(a, sizeof(a), "%.17g", ts.aa);
+    snprintf(z, sizeof(z), "%.17g", ts.z);
+    snprintf(ns, sizeof(ns), "%zu", n);
+    snprintf(st, sizeof(st), "%d", step);
+    const char* py = getenv("HGPM_PYTHON") ? getenv("HGPM_PYTHON") : "python3";
+    std::vector<char*> args = {(char*)py, (char*)p.analysis_py, (char*)fn.c_str(), ns, (char*)"7", st, a, z,
+                               (char*)dir.c_str(), nullptr};
+    pid_t pid;
+    int rc = posix_spawnp(&pid, py, nullptr, nullptr, args.data(), environ);
+    int status = 0;
+    if (rc != 0 || waitpid(pid, &status, 0) < 0 || !WIFEXITED(status) || WEXITSTATUS(status) != 0) {
+        fprintf(stderr, "haccgpm: analysis script %s failed at step %d\n", p.analysis_py, step);
         exit(1);
     }
 }
@@ -98,17 +145,25 @@ int main(int argc, char** argv) {
 
     const int n = p.last_step;
     std::vector<unsigned char> out_after(n > 0 ? n : 1, 0);
-    for (int s = 0; s < n; s++) out_after[s] = p.pk_dump[s] || p.particle_dump[s];
+    auto analysis_at = [&](int s) { return p.do_analysis && p.analysis[s]; };
+    for (int s = 0; s < n; s++) out_after[s] = p.pk_dump[s] || p.particle_dump[s] || analysis_at(s);
     std::vector<hgpm_step_coeffs> co(n > 0 ? n : 1);
     check(hgpm_schedule(&p, n, 1, out_after.data(), co.data()), ctx, "schedule");
 
+    // the timestepper at the end of each step (a and z for the analysis hook): the half
+    // step after the initial conditions, then two per step (driver_pm.cpp:108,117,121)
+    hgpm_timestepper tsa = ts;
+    hgpm_ts_advance_half_step(&p, &tsa);
     const double t_loop = now_s();
     for (int s = 0; s < n; s++) {
         check(hgpm_step(ctx, &co[s]), ctx, "step");
+        hgpm_ts_advance_half_step(&p, &tsa);
+        hgpm_ts_advance_half_step(&p, &tsa);
         if (p.pk_dump[s]) check(hgpm_write_power_spectrum(ctx, (pre + ".pk." + std::to_string(s)).c_str()), ctx,
                                 "GetPowerSpectrum");
         if (p.particle_dump[s])
             check(hgpm_write_output(ctx, (pre + ".particles." + std::to_string(s)).c_str()), ctx, "writeOutput");
+        if (analysis_at(s)) analysis_step(ctx, p, s, tsa);
         if ((s + 1) % 10 == 0 || s + 1 == n) {
             check(hgpm_synchronize(ctx), ctx, "synchronize");
             double el = now_s() - t_loop;

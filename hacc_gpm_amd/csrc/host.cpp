@@ -86,6 +86,10 @@ int hgpm_read_params(const char* fname, hgpm_params* out) {
         else if (!strcmp(key, "PK_BINS")) p.pk_bins = atoi(val);
         else if (!strcmp(key, "PK_DUMP")) steps(p.pk_dump);
         else if (!strcmp(key, "PARTICLE_DUMP")) steps(p.particle_dump);
+        else if (!strcmp(key, "ANALYSIS_STEPS")) steps(p.analysis);
+        else if (!strcmp(key, "ANALYSIS_DIR")) snprintf(p.analysis_dir, sizeof(p.analysis_dir), "%s", val);
+        else if (!strcmp(key, "ANALYSIS_PY")) snprintf(p.analysis_py, sizeof(p.analysis_py), "%s", val);
+        else if (!strcmp(key, "DO_ANALYSIS")) p.do_analysis = 1;
         else if (!strcmp(key, "DUMP_INIT")) p.dump_init = 1;
         else if (!strcmp(key, "DUMP_FINAL")) p.dump_final = 1;
         else if (!strcmp(key, "IPK")) snprintf(p.ipk, sizeof(p.ipk), "%s", val);

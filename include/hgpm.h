@@ -78,6 +78,12 @@ typedef struct hgpm_params {
     int dump_init, dump_final;
     double ol;
     int precision;                        /* HGPM_FP32 (default) or HGPM_FP64 (key PRECISION) */
+    /* Python analysis hook keys (params.cpp:195-214): ANALYSIS_STEPS ("all" or a comma
+       list), ANALYSIS_DIR, ANALYSIS_PY, DO_ANALYSIS (a flag) */
+    unsigned char analysis[HGPM_MAX_STEPS];
+    char analysis_dir[300];
+    char analysis_py[300];
+    int do_analysis;
 } hgpm_params;
 
 /* HACCGPM::Timestepper (src/haccgpm.hpp:178-197) */
@@ -180,6 +186,14 @@ int hgpm_copy_particles(hgpm_ctx* dst, hgpm_ctx* src);
    position difference of one component.  No reference counterpart (accuracy
    diagnostics of FP32 runs against FP64 ones). */
 int hgpm_compare_particles(hgpm_ctx* a, hgpm_ctx* b, double* out);
+
+/* The particle state handed to the Python analysis hook at an analysis step
+   (serial::PyAnalysis, src/analysis/cuda/py_analysis.cu:3-26): 7 floats per particle,
+   (index, x, y, z, vx, vy, vz) in the reference's particle order, the index as a float
+   (the .w of the reference's float4 particles, set by placeParticles,
+   place_particles_initializer.cu:27-34).  out holds 7 * np^3 floats (the reference
+   copies ng^3 entries, which equals np^3 only when NP = NG). */
+int hgpm_analysis_particles(hgpm_ctx* ctx, float* out);
 
 /* serial::GenerateDisplacementIC (src/initial_conditions/initializer.cpp:65-166)
    with the Philox4x32-10 + Box-Muller white noise that replaces cuRAND. ipk is the

@@ -671,6 +671,60 @@ def test_driver_end_to_end(hg, tmp_path):
     assert pos.min() >= 0 and pos.max() < 32
 
 
+def test_driver_analysis_hook(hg, tmp_path):
+    """DO_ANALYSIS with ANALYSIS_STEPS 2,5 (params.cpp:195-214, driver_pm.cpp:153): at
+    those steps and only there the driver writes the 7-float-per-particle array of
+    serial::PyAnalysis (py_analysis.cu:3-26) and runs ANALYSIS_PY on it with the step,
+    a and z after the step.  The step-5 array holds the index and the same float3
+    positions and velocities as the step-5 snapshot."""
+    case("driver analysis hook 32^3 x 7")
+    import json
+    import subprocess
+    p = hg.make_params(32, 32, 64.0, 50.0, 0.0, 100)
+    ipk = tmp_path / "eh.ipk"
+    hg.write_ipk_eh(p, str(ipk), 20.0, 20001)
+    adir = tmp_path / "ana"
+    adir.mkdir()
+    script = tmp_path / "ana.py"
+    script.write_text(
+        "import json, sys\nimport numpy as np\n"
+        "fn, n, w, step, a, z, d = sys.argv[1:8]\n"
+        "x = np.fromfile(fn, np.float32).reshape(int(n), int(w))\n"
+        "json.dump({'n': int(n), 'w': int(w), 'step': int(step), 'a': float(a), 'z': float(z), 'dir': d,\n"
+        "           'index_ok': bool((x[:, 0] == np.arange(int(n))).all()), 'pmin': float(x[:, 1:4].min()),\n"
+        "           'pmax': float(x[:, 1:4].max())}, open(d + '/seen.' + step + '.json', 'w'))\n")
+    prm = tmp_path / "run.params"
+    prm.write_text("\n".join([
+        "OMEGA_CDM 0.261", "DEUT 0.0226576", "OMEGA_NU 0.0", "HUBBLE 0.68", "SS8 0.81", "NS 0.96",
+        "W_DE -1.0", "WA_DE 0.0", "T_CMB 2.726", "N_EFF_MASSLESS 3.04", "N_EFF_MASSIVE 0.0",
+        f"OUTPUT_BASE_NAME {tmp_path}/out", f"IPK {ipk}", "Z_IN 50", "Z_FIN 0", "N_STEPS 100", "LAST_STEP 7",
+        "PK_BINS 32", "PARTICLE_DUMP 5", "ANALYSIS_STEPS 2,5", f"ANALYSIS_DIR {adir}", f"ANALYSIS_PY {script}",
+        "DO_ANALYSIS", "NG 32", "NP 32", "RL 64", "SEED 5", ""]))
+    env = dict(os.environ, HGPM_PYTHON=sys.executable)
+    r = subprocess.run([hg.DRIVER_PATH, str(prm)], capture_output=True, text=True, timeout=120, env=env)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert sorted(f.name for f in adir.iterdir()) == ["out.analysis.2", "out.analysis.5", "seen.2.json",
+                                                      "seen.5.json"]
+    ts = hg.Timestepper(p)
+    ts.setInitialZ(p.z_ini)
+    ts.reverseHalfStep()
+    ts.advanceHalfStep()
+    for step in range(6):
+        ts.advanceHalfStep()
+        ts.advanceHalfStep()
+        if step in (2, 5):
+            seen = json.load(open(adir / f"seen.{step}.json"))
+            assert seen["n"] == 32 ** 3 and seen["w"] == 7 and seen["step"] == step and seen["index_ok"]
+            assert seen["a"] == ts.aa and seen["z"] == ts.z and seen["dir"] == str(adir)
+            assert 0 <= seen["pmin"] and seen["pmax"] < 32
+    ana = np.fromfile(adir / "out.analysis.5", np.float32).reshape(-1, 7)
+    snap = np.fromfile(tmp_path / "out.particles.5", np.float32)
+    n = 32 ** 3
+    assert np.array_equal(ana[:, 1:4], snap[:3 * n].reshape(n, 3))
+    assert np.array_equal(ana[:, 4:7], snap[3 * n:].reshape(n, 3))
+
+
 # -------------------------------------------------------------- ensemble ----
 def _ic_inputs(hg, p):
     import tempfile

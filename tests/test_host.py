@@ -40,6 +40,18 @@ def test_read_reference_params_fixture(hg):
     assert dumps == [0, 1, 3, 5, 10, 20, 30, 40, 59, 66, 74, 86, 101, 122, 153, 175, 205, 247, 310, 415, 610, 624]
     assert p.prefix == b"runs/test3" and p.pk_bins == 221 and p.precision == hg.FP32
     assert not any(p.particle_dump) and p.dump_init == 0
+    # analysis hook keys (params.cpp:195-214): "ANALYSIS_STEPS all", ANALYSIS_PY set,
+    # ANALYSIS_DIR and DO_ANALYSIS commented out
+    assert all(p.analysis) and p.analysis_py == b"/home/hqureshi/HACC_GPM/example_analysis/basic"
+    assert p.analysis_dir == b"" and p.do_analysis == 0
+
+
+def test_read_params_analysis_keys(hg, tmp_path):
+    f = tmp_path / "a.params"
+    f.write_text("NG 16\nNP 16\nANALYSIS_STEPS 0,7,999\nANALYSIS_DIR /tmp/x\nANALYSIS_PY mod.py\nDO_ANALYSIS\n")
+    p = hg.read_params(str(f))
+    assert [i for i in range(1000) if p.analysis[i]] == [0, 7, 999]
+    assert p.analysis_dir == b"/tmp/x" and p.analysis_py == b"mod.py" and p.do_analysis == 1
 
 
 def test_save_params_roundtrip(hg, tmp_path):
