@@ -170,11 +170,11 @@ def test_deposit_heavy_particles(hg, orc, ng, np_, sigma):
         assert np.array_equal(m.get_density(), ref)
         wide, deferred = m.get_tunable("deposit_wide"), m.get_tunable("deposit_deferred")
         case(f"  routes: wide {wide} deferred {deferred}")
+        # a tile's window spans 7^3 to 9^3 owner blocks: those over the 512-chunk limit
+        # take the atomic route (every tile at 256/32), and heavy cells the 64-bit pass
         if ng == 192 and sigma > 1:
-            assert deferred == 0  # 7^3 owner blocks per tile at most: every tile on the arena
+            assert deferred < m.N // 4096
         else:
-            # at 256/32 a tile's window spans up to 9^3 owner blocks (over the 512-chunk
-            # limit: the atomic route); heavy cells take the 64-bit pass
             assert wide + deferred > 0
         m.set_tunable("deposit_arena", 0)  # every tile on the atomic route
         m.cic()

@@ -114,11 +114,15 @@ def _ic(hg, p, k_max=100.0):
 
 
 def _fp64_pk_of_box(hg, p, pos, vel, n_steps):
-    """The FP64 plain path from a box's float3 initial state (grid units), whole schedule."""
+    """The FP64 plain path from a box's float3 initial state (grid units), whole schedule.
+    A float3 position c + f rounds up to ng for a cell ng - 1 with f close to 1: it is
+    wrapped to 0, the same point of the periodic box."""
     q = hg.make_params(p.ng, p.np, p.rl, p.z_ini, 0.0, p.nsteps, seed=p.seed, ss8=p.ss8,
                        pk_bins=p.pk_bins, precision=hg.FP64)
+    x = pos.astype(np.float64)
+    x[x >= p.ng] -= p.ng
     with hg.MemoryManager(q) as m:
-        m.set_particles_f64(pos.astype(np.float64), vel.astype(np.float64))
+        m.set_particles_f64(x, vel.astype(np.float64))
         for co in hg.schedule(q, n_steps, merge=False):
             m.step(co)
         return m.power_spectrum(0)
