@@ -805,10 +805,13 @@ __global__ __launch_bounds__(256) void k_deposit_total(const unsigned long long*
 // force grid, so both give the same bits.
 __device__ __forceinline__ float central_diff(float up, float down) { return up - down; }
 
-// Gather of the force with the deposit's CIC weights, v += float(w*deltaT*fscal)*F
-// summed over the 8 corners in (x,y,z) order, then optionally the drift.  The force of
-// each corner is the central difference of the potential (central_diff of psi, the
-// stored half potential).  A
+// Gather of the force with the deposit's CIC weights and the kick (ICICKernel,
+// icic_pm.cu:72-127): S = sum_c w_c F_c over the 8 corners in (x,y,z) order, then
+// v += float(deltaT*fscal) S (the reference rounds a factor float(w*deltaT*fscal) per
+// corner in FP64 instead; this form needs one FP32 rounding of the factor and no FP64
+// work per particle), then optionally the drift.  The force of each corner is the
+// central difference of the potential (central_diff of psi, the stored half
+// potential).  A
 // workgroup takes GT_TILE consecutive particles (part of a Lagrangian tile, a compact
 // block of the lattice).  The potential under the bounding box of the particles' base
 // cells, widened by one cell below and two above (the corners and their neighbours),
@@ -842,6 +845,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     __shared__ unsigned long long msk[2 * GT_MASKW]; // row mask of a sparse slab: bm, need
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
+    // the kick factor deltaT * fscal (icic_pm.cu:110) rounded once to FP32
+    const float kc = (float)(deltaT * fscal);
     // the potential of this tile's box (a batch holds several boxes of the same shape)
     psi += (p0 / npb) * gcells;
     // BOXED: the deposit of these positions stored the box of each half deposit tile
@@ -1054,7 +1059,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                         zm[u][v] = sp[b0 + u * sxs + v * ezp - 1]; // (x+u, y+v, z-1)
                         zp[u][v] = sp[b0 + u * sxs + v * ezp + 2]; // (x+u, y+v, z+2)
                     }
-                float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+                // S = sum_c w_c F_c (fmaf, corners in (x, y, z) order), then v += k S
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
                 for (int c = 0; c < 8; c++) {
                     const int cx = c >> 2, cy = (c >> 1) & 1, cz = c & 1;
@@ -1063,13 +1069,12 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                     const float fz = cz ? central_diff(zp[cx][cy], q[cx][cy][0]) : central_diff(q[cx][cy][1], zm[cx][cy]);
                     float w = gx[cx] * gy[cy];
                     w = w * gz[cz];
-                    const float mul = (float)(((double)w * deltaT) * fscal);
-                    float t;
-                    t = mul * fx; dvx = dvx + t;
-                    t = mul * fy; dvy = dvy + t;
-                    t = mul * fz; dvz = dvz + t;
+                    sx = __builtin_fmaf(w, fx, sx);
+                    sy = __builtin_fmaf(w, fy, sy);
+                    sz = __builtin_fmaf(w, fz, sz);
                 }
-                const float vx = V[k][0] + dvx, vy = V[k][1] + dvy, vz = V[k][2] + dvz;
+                const float vx = __builtin_fmaf(kc, sx, V[k][0]), vy = __builtin_fmaf(kc, sy, V[k][1]),
+                            vz = __builtin_fmaf(kc, sz, V[k][2]);
                 vel[3 * j] = vx;
                 vel[3 * j + 1] = vy;
                 vel[3 * j + 2] = vz;

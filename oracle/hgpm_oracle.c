@@ -491,13 +491,20 @@ void orc_solve_gradient(int ng, int mode, const double* rho, double* force) {
 /* ---------------------------------------------------------- gather/kick ---- */
 /* ICICKernel (src/pm_actions/cuda/icic_pm.cu:72-127) in the build's split layout:
    w = (dx*dy)*dz (FP32), mul = float((double)w*deltaT*fscal), dV += mul*F (FP32). */
+/* The build's FP32 kick (SPLIT mode).  The reference forms a kick factor per corner,
+   float(w * deltaT * fscal) in FP64 (icic_pm.cu:110), and adds factor * F per corner.
+   Here the weights' sum S = sum_c w_c F_c is formed first (fmaf per corner, corners in
+   (x, y, z) order, w = (dx * dy) * dz as the deposit forms it), then v = fmaf(k, S, v)
+   with k = float(deltaT * fscal): the same quantity with one FP32 rounding of the
+   factor instead of eight, and no FP64 work per particle. */
 void orc_kick_split(int ng, int64_t n, const int32_t* cell, const float* off, float* vel,
                     const float* force, double deltaT, double fscal) {
+    const float kc = (float)(deltaT * fscal);
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; p++) {
         int i = cell[3 * p], j = cell[3 * p + 1], k = cell[3 * p + 2];
         float fx = off[3 * p], fy = off[3 * p + 1], fz = off[3 * p + 2];
-        float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         for (int x = 0; x < 2; x++)
             for (int y = 0; y < 2; y++)
                 for (int z = 0; z < 2; z++) {
@@ -506,16 +513,14 @@ void orc_kick_split(int ng, int64_t n, const int32_t* cell, const float* off, fl
                     float dz = z ? fz : 1.0f - fz;
                     float w = dx * dy;
                     w = w * dz;
-                    float mul = (float)(((double)w * deltaT) * fscal);
                     const float* F = force + 3 * cidx(ng, (i + x) % ng, (j + y) % ng, (k + z) % ng);
-                    float t;
-                    t = mul * F[0]; dvx = dvx + t;
-                    t = mul * F[1]; dvy = dvy + t;
-                    t = mul * F[2]; dvz = dvz + t;
+                    sx = fmaf(w, F[0], sx);
+                    sy = fmaf(w, F[1], sy);
+                    sz = fmaf(w, F[2], sz);
                 }
-        vel[3 * p] = vel[3 * p] + dvx;
-        vel[3 * p + 1] = vel[3 * p + 1] + dvy;
-        vel[3 * p + 2] = vel[3 * p + 2] + dvz;
+        vel[3 * p] = fmaf(kc, sx, vel[3 * p]);
+        vel[3 * p + 1] = fmaf(kc, sy, vel[3 * p + 1]);
+        vel[3 * p + 2] = fmaf(kc, sz, vel[3 * p + 2]);
     }
 }
 
