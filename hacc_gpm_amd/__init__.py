@@ -27,6 +27,7 @@ DRIVER_PATH = os.path.join(_HERE, "bin", "haccgpm")
 OK, ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_IO, ERR_UNSUPPORTED, ERR_STATE, ERR_NUMERIC = range(8)
 FP32, FP64 = 0, 1
 MAX_STEPS = 1000
+IC_STATIC_BINS = 262144  # HGPM_IC_STATIC_BINS: largest .ipk table of generate_ic_noalloc
 MASS_UNIT = 2.0 ** -24
 
 
@@ -115,6 +116,8 @@ def lib():
             "hgpm_analysis_particles": (c_int, [vp, vp]),
             "hgpm_compare_particles": (c_int, [vp, vp, vp]),
             "hgpm_generate_ic": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double, c_double]),
+            "hgpm_generate_ic_noalloc": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double,
+                                                 c_double]),
             "hgpm_update_positions": (c_int, [vp, c_double]),
             "hgpm_cic": (c_int, [vp]),
             "hgpm_get_density": (c_int, [vp, vp]),
@@ -388,6 +391,14 @@ class MemoryManager:
         v = np.ascontiguousarray(ipk_values, np.float64)
         self._c(lib().hgpm_generate_ic(self.h, _ptr(v), int(v.size), float(ipk_header[3]), float(ipk_header[1]),
                                        float(delta), float(dot_delta), float(fscal), float(deltaT)))
+
+    def generate_ic_noalloc(self, ipk_header, ipk_values, delta, dot_delta, fscal, deltaT):
+        """generate_ic with the .ipk table staged in the device's fixed array (no device
+        allocation; tables of at most IC_STATIC_BINS bins)."""
+        v = np.ascontiguousarray(ipk_values, np.float64)
+        self._c(lib().hgpm_generate_ic_noalloc(self.h, _ptr(v), int(v.size), float(ipk_header[3]),
+                                               float(ipk_header[1]), float(delta), float(dot_delta), float(fscal),
+                                               float(deltaT)))
 
     def update_positions(self, prefactor):
         self._c(lib().hgpm_update_positions(self.h, float(prefactor)))

@@ -41,6 +41,7 @@ struct hgpm_ctx {
     int nstore = 1, nwave = 1;
     hipStream_t st = nullptr;
     int device = 0;
+    double* ic_table = nullptr; // the device's fixed .ipk staging array (hgpm_generate_ic_noalloc)
     std::string err;
     size_t bytes = 0;
     const FftOps* fft = nullptr;    // length ng
@@ -454,6 +455,12 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
         c->err = "hipStreamCreate failed";
         return bail(HGPM_ERR_HIP);
     }
+    // set-up of the device's fixed staging array (once per device, this device current)
+    if (ic_static_table(c->device, &c->ic_table) != 0) {
+        (void)hipGetLastError();
+        c->err = "hipGetSymbolAddress of the .ipk staging array failed";
+        return bail(HGPM_ERR_HIP);
+    }
     const size_t N = (size_t)c->N, G = (size_t)c->G;
     if (c->prec == HGPM_FP32) {
         // particles for every stored box; every other per-box buffer for one wave
@@ -835,9 +842,10 @@ int hgpm_compare_particles(hgpm_ctx* a, hgpm_ctx* b, double* out) {
 }
 
 // -------------------------------------------------------------------- IC ----
-int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_delta, double ipk_k_max,
-                     double delta, double dot_delta, double fscal, double deltaT) {
-    if (!c || !ipk || ipk_bins < 2 || !(ipk_k_delta > 0)) return fail(c, HGPM_ERR_ARG, "hgpm_generate_ic: bad arguments");
+namespace {
+// Argument checks shared by the two IC entry points.
+int ic_check(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_delta, double ipk_k_max, const char* who) {
+    if (!c || !ipk || ipk_bins < 2 || !(ipk_k_delta > 0)) return fail(c, HGPM_ERR_ARG, std::string(who) + ": bad arguments");
     const int np = c->np;
     // launch_interpolate_pk range check (interpolation_initializer.cu:68-74), kept as is
     double maxK = ((np / 2) * 2 * HGPM_PI_D) / c->p.rl;
@@ -846,14 +854,18 @@ int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_
     // the interpolation reads bin (int)(k/k_delta)+1 for |k| up to sqrt(3) pi np/rl
     double kmax_grid = sqrt(3.0) * (np / 2) * 2 * HGPM_PI_D / c->p.rl;
     if ((int)(kmax_grid / ipk_k_delta) + 1 >= ipk_bins) return fail(c, HGPM_ERR_ARG, "ipk table too short for this grid");
+    return HGPM_OK;
+}
+
+// The IC kernels from a device copy of the .ipk table, d_ipk, on the context's stream;
+// returns after the stream has drained.  Allocates nothing.
+int generate_ic_from(hgpm_ctx* c, const double* d_ipk, int ipk_bins, double ipk_k_delta, double delta,
+                     double dot_delta, double fscal, double deltaT) {
     c->boxes_valid = false;
-    double* d_ipk = nullptr;
-    int rc = dalloc(c, &d_ipk, sizeof(double) * ipk_bins, "ipk");
-    if (rc) return rc;
-    c->bytes -= sizeof(double) * ipk_bins;
-    HIPCHK(c, hipMemcpyAsync(d_ipk, ipk, sizeof(double) * ipk_bins, hipMemcpyHostToDevice, c->st));
+    const int np = c->np;
     const double a = 1.0 / (1.0 + c->p.z_ini);
     const uint32_t seed = (uint32_t)c->p.seed;
+    int rc = HGPM_OK;
     if (c->prec == HGPM_FP32) {
         // reference float rounding of the velocity factor (place_particles_initializer.cu:200-205)
         float velA = (float)(a - (deltaT * 0.5f));
@@ -881,10 +893,38 @@ int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_
     }
     LAUNCHCHK(c);
     HIPCHK(c, hipStreamSynchronize(c->st));
-    (void)hipFree(d_ipk);
     c->have_particles = true;
     c->have_density = c->have_force = false;
     return HGPM_OK;
+}
+} // namespace
+
+int hgpm_generate_ic(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_delta, double ipk_k_max,
+                     double delta, double dot_delta, double fscal, double deltaT) {
+    int rc = ic_check(c, ipk, ipk_bins, ipk_k_delta, ipk_k_max, "hgpm_generate_ic");
+    if (rc) return rc;
+    double* d_ipk = nullptr;
+    if ((rc = dalloc(c, &d_ipk, sizeof(double) * ipk_bins, "ipk"))) return rc;
+    c->bytes -= sizeof(double) * ipk_bins;
+    hipError_t e = hipMemcpyAsync(d_ipk, ipk, sizeof(double) * ipk_bins, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) rc = generate_ic_from(c, d_ipk, ipk_bins, ipk_k_delta, delta, dot_delta, fscal, deltaT);
+    else rc = fail(c, HGPM_ERR_HIP, std::string("hgpm_generate_ic: ") + hipGetErrorString(e));
+    (void)hipStreamSynchronize(c->st);
+    (void)hipFree(d_ipk);
+    return rc;
+}
+
+int hgpm_generate_ic_noalloc(hgpm_ctx* c, const double* ipk, int ipk_bins, double ipk_k_delta, double ipk_k_max,
+                             double delta, double dot_delta, double fscal, double deltaT) {
+    int rc = ic_check(c, ipk, ipk_bins, ipk_k_delta, ipk_k_max, "hgpm_generate_ic_noalloc");
+    if (rc) return rc;
+    if (ipk_bins > HGPM_IC_STATIC_BINS)
+        return fail(c, HGPM_ERR_ARG, "hgpm_generate_ic_noalloc: the .ipk table has " + std::to_string(ipk_bins) +
+                                         " bins, more than the " + std::to_string(HGPM_IC_STATIC_BINS) +
+                                         " of the fixed staging array");
+    if (!c->ic_table) return fail(c, HGPM_ERR_STATE, "hgpm_generate_ic_noalloc: no staging array on this device");
+    HIPCHK(c, hipMemcpyAsync(c->ic_table, ipk, sizeof(double) * ipk_bins, hipMemcpyHostToDevice, c->st));
+    return generate_ic_from(c, c->ic_table, ipk_bins, ipk_k_delta, delta, dot_delta, fscal, deltaT);
 }
 
 // ------------------------------------------------------------ step parts ----

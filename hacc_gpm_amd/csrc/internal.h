@@ -151,6 +151,9 @@ void launch_ic_place_split(const float2* s_grid, PosPackH* pos, float* vel, int 
                            double delta, float vel_mul, hipStream_t s);
 void launch_ic_place_f64(const double2* s_grid, double* pos, double* vel, int np, int ng, int dim,
                          double delta, double vel_mul, hipStream_t s);
+// Device address of the fixed .ipk staging array (HGPM_IC_STATIC_BINS doubles) on
+// `device`, the current device; 0 on success.
+int ic_static_table(int device, double** ptr);
 
 // ---- FP64 plain path (k_fp64.hip) ----
 void launch_f64_drift(double* pos, const double* vel, int64_t np3, int ng, double pf, hipStream_t s);

@@ -13,6 +13,9 @@
 //                  inverse transform and the 1/np^3 ScaleFFT.
 #include "common.h"
 #include "internal.h"
+#include "../../include/hgpm.h"
+
+#include <mutex>
 
 #pragma clang fp contract(off)
 
@@ -264,5 +267,35 @@ void launch_ic_place_split(const float2* sg, PosPackH* pos, float* vel, int np, 
 void launch_ic_place_f64(const double2* sg, double* pos, double* vel, int np, int ng, int dim, double delta,
                          double vel_mul, hipStream_t s) {
     k_place_f64<<<nblk((int64_t)np * np * np), BS, 0, s>>>(sg, pos, vel, np, ng, dim, delta, vel_mul);
+}
+
+} // namespace hgpm
+
+// ---- staging of the .ipk table for hgpm_generate_ic_noalloc ----
+// A fixed device array in place of the table buffer hgpm_generate_ic allocates per
+// call.  Each device holds its own copy from the first context created on it until the
+// process ends.  Host code never passes this name to a kernel or a copy (in host code it
+// is a stand-in, not a device address): it uses the address ic_static_table returns.
+namespace {
+__device__ __align__(256) double scratch[HGPM_IC_STATIC_BINS];
+
+constexpr int IC_MAX_DEVICES = 64;
+std::mutex g_ic_mu;
+double* g_ic_ptr[IC_MAX_DEVICES] = {};
+} // namespace
+
+namespace hgpm {
+// The device address of the staging array on `device`, which must be the current
+// device: looked up once per device (hipGetSymbolAddress), then cached.
+int ic_static_table(int device, double** ptr) {
+    if (device < 0 || device >= IC_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(g_ic_mu);
+    if (!g_ic_ptr[device]) {
+        double* p = nullptr;
+        if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        g_ic_ptr[device] = p;
+    }
+    *ptr = g_ic_ptr[device];
+    return 0;
 }
 } // namespace hgpm

@@ -204,6 +204,20 @@ int hgpm_generate_ic(hgpm_ctx* ctx, const double* ipk, int ipk_bins, double ipk_
                      double ipk_k_max, double delta, double dot_delta, double fscal,
                      double deltaT);
 
+/* hgpm_generate_ic that allocates no device memory.  No counterpart in the reference: it
+   is for callers that cannot allocate (code under HIP graph capture, serving loops under
+   a no-allocation policy).  The .ipk table is staged in a fixed device array of
+   HGPM_IC_STATIC_BINS doubles (2 MiB) instead of a buffer allocated per call; each
+   device holds one such array from the first context created on it until the process
+   ends.  Tables of more than HGPM_IC_STATIC_BINS bins are refused (HGPM_ERR_ARG).
+   Results are bit-identical to hgpm_generate_ic.  Every context of a device shares the
+   array and each call overwrites it: calls that overlap on one device (contexts driven
+   from several host threads) race.  Per-kernel timing (hgpm_set_timing) creates HIP
+   events; leave it off where nothing may be allocated. */
+#define HGPM_IC_STATIC_BINS 262144
+int hgpm_generate_ic_noalloc(hgpm_ctx* ctx, const double* ipk, int ipk_bins, double ipk_k_delta,
+                             double ipk_k_max, double delta, double dot_delta, double fscal, double deltaT);
+
 /* ----------------------------------------------------- per-step primitives ---- */
 /* serial::UpdatePositions (src/pm_actions/pm_actions.cpp:287-314): x += v*prefactor,
    periodic wrap. prefactor = dt/(a^2 adot)*frac, rounded to float as the reference. */
