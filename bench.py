@@ -22,7 +22,10 @@ Also reported on the same line:
                  in FP64, on all host threads and on one, per-stage seconds;
   ensemble     : 96 boxes (32 each of 64^3, 128^3 and 192^3 particles, per-box seed and
                  sigma_8) stepped 40 steps z = 30 -> 0 as a batched ensemble (HIP graph
-                 runs) and one box at a time, and the speed-up.
+                 runs) and one box at a time, and the speed-up;
+  pressure_ensemble : 24 boxes of 384^3 particles on 768^3 grids with 200 of 96^3,
+                 30 steps z = 40 -> 0, in 90% of the free device memory (the large
+                 batch in waves).
 """
 import argparse
 import json
@@ -277,7 +280,7 @@ def pressure_boxes(hg, n_big=24, n_small=200, z_ini=40.0, n_steps=30):
     return ps
 
 
-def run_pressure(hg, steps, warmup, n_big, n_small, budget):
+def run_pressure(hg, steps, warmup, n_big, n_small, budget, classes_alone=True):
     """The memory-pressure ensemble in one device budget (0: 90% of the free memory):
     batches that do not fit keep grid buffers for a wave of their boxes and step the
     waves in turn.  Then each size class alone in the same budget."""
@@ -316,8 +319,10 @@ def run_pressure(hg, steps, warmup, n_big, n_small, budget):
 
     progress(f"{len(ps)} boxes in one ensemble")
     mixed = one(ps)
-    progress("each size class alone")
-    alone = [dict(one([p for p in ps if key(p) == k]), np=k[1], ng=k[0]) for k in sorted(ics)]
+    alone = []
+    if classes_alone:
+        progress("each size class alone")
+        alone = [dict(one([p for p in ps if key(p) == k]), np=k[1], ng=k[0]) for k in sorted(ics)]
     nparts = sum(p.np ** 3 for p in ps)
     return {"metric": "aggregate particle-steps/s, ensemble under memory pressure",
             "value": mixed["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -435,6 +440,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
+    ap.add_argument("--no-pressure", action="store_true", help="skip the memory-pressure ensemble")
     ap.add_argument("--no-full-run", action="store_true",
                     help="skip the whole z_ini -> z_fin run timed step by step")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -589,6 +595,15 @@ def main():
                                                    "speedup", "batches", "config")}
         except Exception as e:  # report, do not hide
             out["ensemble"] = {"error": str(e)}
+    if not args.no_pressure:
+        # the memory-pressure workload (24 x 384^3 on 768^3 grids with 200 x 96^3, 30 steps
+        # z = 40 -> 0) in 90% of the free device memory: batches stepped in waves
+        progress("pressure ensemble: 24 x 384^3 on 768^3 + 200 x 96^3")
+        try:
+            pr = run_pressure(hg, 27, 3, 24, 200, 0, classes_alone=False)
+            out["pressure_ensemble"] = {k: pr[k] for k in ("value", "unit", "ms_per_step", "mixed", "config")}
+        except Exception as e:  # report, do not hide
+            out["pressure_ensemble"] = {"error": str(e)}
     if not args.no_cpu:
         th = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         progress(f"CPU baseline on {th} threads")
