@@ -4,7 +4,8 @@
 // x-major ng^3 grids) and fuses the spectral solve of src/pm_solver/solver.cpp:145-170
 // into the transform passes.
 //
-// Passes (grid index (x*N + y)*PZ + z, z contiguous; PZ = N for full complex grids):
+// Passes (grid index (x*N + y)*PZ + z, z contiguous; PZ = N for full complex grids; half
+// spectra add HPAD elements after each x plane):
 //   rows  : 1-D transforms along z; a workgroup loads RC whole rows (coalesced).
 //   cols  : 1-D transforms along y (stride PZ) or x (stride N*PZ); a workgroup loads CC
 //           adjacent z values of N rows (CC*8 contiguous bytes per row).
@@ -33,6 +34,15 @@ constexpr int N = HGPM_FFT_N;
 constexpr int NT = 256;   // rows passes
 constexpr int HZ = N / 2 + 16; // half-spectrum row pitch: k_z = 0..N/2 + zero padding
 static_assert(HZ % 16 == 0 && HZ <= N, "half-spectrum pitch");
+// Half-spectrum x-plane pitch: N rows of HZ plus HPAD unused elements.  Without the pad
+// the plane stride N HZ 8 bytes is a multiple of 2^17 (at N = 1024), so the N rows an
+// x-pass tile reads (one per x) share their low address bits; the pad shifts each
+// plane by one 128-byte line.  (N = 32 has no room for it: there HZ = N.)
+constexpr int HPAD = N >= 64 ? 16 : 0;
+constexpr size_t NPH = (size_t)N * HZ + HPAD;
+static_assert(HPAD % 16 == 0 && (size_t)N * NPH <= (size_t)N * N * N, "half spectra fit a full grid");
+// element offset of half-spectrum row r = (box N + x) N + y
+__device__ __forceinline__ size_t hrow(size_t r) { return (r / N) * NPH + (r % N) * HZ; }
 
 // rows per row tile / columns per column tile (FP32, FP64)
 constexpr int rc32() { return N >= 768 ? 4 : (N >= 384 ? 8 : (N >= 192 ? 16 : (N >= 96 ? 32 : 64))); }
@@ -71,7 +81,8 @@ __global__ __launch_bounds__(NT) void k_rows_density(const long long* __restrict
 // axis 1 (y): tile = (x, z-chunk), element (n=y, c) at x N^2 + n N + z0 + c
 // axis 0 (x): tile = (y, z-chunk), element (n=x, c) at n N^2 + y N + z0 + c
 template <typename V, int CC, int NTC, int DIR>
-__global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis, int pz) {
+__global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __restrict tw, int axis, int pz,
+                                                  size_t plane) {
     __shared__ V lds[lds_elems<N, CC>()];
     // adjacent z-chunks of one row share 128-byte lines: keep them on one XCD's L2
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
@@ -79,7 +90,7 @@ __global__ __launch_bounds__(NTC) void k_cols_c2c(V* __restrict d, const V* __re
     // outer runs over (box, x) for the y pass and (box, y) for the x pass: a batch holds
     // several grids of the same shape back to back
     const int outer = tile / nzc, zc = tile % nzc;
-    const size_t NP = (size_t)N * pz;
+    const size_t NP = plane; // x-plane pitch (N pz for full grids, NPH for half spectra)
     const size_t base = axis == 1 ? (size_t)outer * NP + (size_t)zc * CC
                                   : (size_t)(outer / N) * N * NP + (size_t)(outer % N) * pz + (size_t)zc * CC;
     const size_t stride = axis == 1 ? (size_t)pz : NP;
@@ -125,8 +136,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c_density(const long long* __rest
             a = make_float2(0.5f * (x.x + y.x), 0.5f * (x.y - y.y));
             b = make_float2(0.5f * (x.y + y.y), 0.5f * (y.x - x.x));
         }
-        out[(row0 + 2 * c) * HZ + k] = a;
-        out[(row0 + 2 * c + 1) * HZ + k] = b;
+        out[hrow(row0 + 2 * c) + k] = a;
+        out[hrow(row0 + 2 * c + 1) + k] = b;
     }
 }
 
@@ -145,7 +156,7 @@ __global__ __launch_bounds__(NTC) void k_x_psi(float2* __restrict h1, const floa
     const int tile = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int outer = tile / (HZ / CC), zc = tile % (HZ / CC); // outer = (box, y)
     const int y = outer % N;
-    const size_t NP = (size_t)N * HZ;
+    const size_t NP = NPH;
     const size_t base = (size_t)(outer / N) * N * NP + (size_t)y * HZ + (size_t)zc * CC;
     typedef LastShape<N, CC, NTC> LS;
     typedef StageShape<N, CC, NTC, LS::R> SS;
@@ -184,7 +195,7 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_real(const float2* __restrict h
     const size_t row0 = (size_t)blockIdx.x * RC;
     fft_tile<float2, N, PC, NT, +1, false, true>(
         [&](int, int, int n, int c) {
-            const float2 a = herm(hz + (row0 + 2 * c) * HZ, n), b = herm(hz + (row0 + 2 * c + 1) * HZ, n);
+            const float2 a = herm(hz + hrow(row0 + 2 * c), n), b = herm(hz + hrow(row0 + 2 * c + 1), n);
             return make_float2(a.x - b.y, a.y + b.x);
         },
         [&](int, int, int n, int c, float2 v) {
@@ -201,10 +212,10 @@ __global__ __launch_bounds__(NT) void k_rows_c2r_real(const float2* __restrict h
 // means fewer elements, and registers, per thread: the fused x pass holds the
 // spectrum between its two transforms.
 template <typename V, int CC, int NTC>
-void cols_launch(V* d, int axis, int dir, const V* tw, int pz, hipStream_t s, int nbox = 1) {
+void cols_launch(V* d, int axis, int dir, const V* tw, int pz, size_t plane, hipStream_t s, int nbox = 1) {
     unsigned nb = (unsigned)((size_t)nbox * N * (pz / CC));
-    if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
-    else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis, pz);
+    if (dir < 0) k_cols_c2c<V, CC, NTC, -1><<<nb, NTC, 0, s>>>(d, tw, axis, pz, plane);
+    else k_cols_c2c<V, CC, NTC, +1><<<nb, NTC, 0, s>>>(d, tw, axis, pz, plane);
 }
 template <typename V, int RC>
 void rows_launch(V* d, int dir, const V* tw, hipStream_t s) {
@@ -240,19 +251,19 @@ void pass_f32(float2* d, int axis, int dir, const float2* tw, int variant, hipSt
     if (axis == 2) { rows_launch<float2, rc32()>(d, dir, tw, s); return; }
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, N, s);
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, N, (size_t)N * N, s);
     });
 }
 // Column pass over nbox half-spectrum grids N x N x HZ.
 void pass_half(float2* d, int axis, int dir, const float2* tw, int variant, int nbox, hipStream_t s) {
     with_variant(variant, [&](auto vc) {
         constexpr int v = decltype(vc)::value;
-        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, HZ, s, nbox);
+        cols_launch<float2, cc_of(v), nt_of(v)>(d, axis, dir, tw, HZ, NPH, s, nbox);
     });
 }
 void pass_f64(double2* d, int axis, int dir, const double2* tw, hipStream_t s) {
     if (axis == 2) rows_launch<double2, rc64()>(d, dir, tw, s);
-    else cols_launch<double2, cc64(), 512>(d, axis, dir, tw, N, s);
+    else cols_launch<double2, cc64(), 512>(d, axis, dir, tw, N, (size_t)N * N, s);
 }
 void fwd_z_density(const long long* rho, const unsigned long long* total, float2* out, const float2* tw,
                    hipStream_t s) {
