@@ -103,13 +103,14 @@ def run_engine(hg, p, steps, warmup, timing=True, gather_tile=-1):
     n_total = warmup + steps
     sched = hg.schedule(p, n_total, merge=True)
     with hg.MemoryManager(p) as m:
-        tune = m.autotune(3) if p.precision == hg.FP32 else None
-        if gather_tile >= 0 and p.precision == hg.FP32:
-            m.set_tunable("gather_tile", gather_tile)
         t0 = time.perf_counter()
         hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
         m.synchronize()
         t_ic = time.perf_counter() - t0
+        # FFT shapes, then the deposit tile and gather shapes on these particles
+        tune = m.autotune(3) if p.precision == hg.FP32 else None
+        if gather_tile >= 0 and p.precision == hg.FP32:
+            m.set_tunable("gather_tile", gather_tile)
         for s in range(warmup):
             m.step(sched[s])
         m.synchronize()
@@ -144,8 +145,8 @@ def run_full(hg, p, per_kernel=True):
     arena = []  # 32-bit arena words reserved per grid cell, per step
     regions = []
     with hg.MemoryManager(p) as m:
-        m.autotune(3)
         hg.GenerateDisplacementIC(m, p, ts, hdr, vals)
+        m.autotune(3)
         m.synchronize()
         for s in range(n):
             m.set_timing(per_kernel)
@@ -294,12 +295,12 @@ def run_pressure(hg, steps, warmup, n_big, n_small, budget, classes_alone=True):
 
     def one(pl):
         with hg.Ensemble(pl, device_budget=budget) as e:
-            e.autotune(2)
             t0 = time.perf_counter()
             for b, p in enumerate(pl):
                 e.generate_ic(b, *ics[key(p)])
             e.synchronize()
             t_ic = time.perf_counter() - t0
+            e.autotune(2)
             for s in range(warmup):
                 e.step([scheds[key(p)][s] for p in pl])
             e.synchronize()
@@ -357,9 +358,9 @@ def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
     scheds = [hg.schedule(p, n_total, merge=True) for p in ps]
     nparts = sum(p.np ** 3 for p in ps)
     with hg.Ensemble(ps) as e:
-        e.autotune(2)
         for b, a in enumerate(ics):
             e.generate_ic(b, *a)
+        e.autotune(2)
         for s in range(warmup):
             e.step([sc[s] for sc in scheds])
         e.synchronize()
@@ -373,9 +374,9 @@ def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
     # the same run with the steps captured into one HIP graph (capture and instantiation
     # inside the timed region)
     with hg.Ensemble(ps) as e:
-        e.autotune(2)
         for b, a in enumerate(ics):
             e.generate_ic(b, *a)
+        e.autotune(2)
         e.run([[sc[s] for sc in scheds] for s in range(warmup)])
         e.synchronize()
         t0 = time.perf_counter()
@@ -384,9 +385,17 @@ def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
         t_batched = time.perf_counter() - t0
     progress("one box at a time")
     t_single = 0.0
+    tuned = {}  # per size class: the autotuner's choices on its first box, kept for the others
+    keys = ("fft_cols", "fft_xsolve", "deposit_tile", "gather_tile")
     for b, p in enumerate(ps):
         with hg.MemoryManager(p) as m:
             m.generate_ic(*ics[b])
+            if p.np not in tuned:
+                m.autotune(2)
+                tuned[p.np] = {k: m.get_tunable(k) for k in keys}
+            else:
+                for k, v in tuned[p.np].items():
+                    m.set_tunable(k, v)
             for s in range(warmup):
                 m.step(scheds[b][s])
             m.synchronize()
@@ -402,9 +411,9 @@ def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
     for n in sorted({p.np for p in ps}) if classes_alone else []:
         idx = [b for b, p in enumerate(ps) if p.np == n]
         with hg.Ensemble([ps[b] for b in idx]) as e:
-            e.autotune(2)
             for j, b in enumerate(idx):
                 e.generate_ic(j, *ics[b])
+            e.autotune(2)
             for s in range(warmup):
                 e.step([scheds[b][s] for b in idx])
             e.synchronize()

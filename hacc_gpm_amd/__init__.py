@@ -116,6 +116,7 @@ def lib():
             "hgpm_analysis_particles": (c_int, [vp, vp]),
             "hgpm_compare_particles": (c_int, [vp, vp, vp]),
             "hgpm_generate_ic": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double, c_double]),
+            "hgpm_autotune_particles": (c_int, [vp, c_int, vp, vp]),
             "hgpm_generate_ic_noalloc": (c_int, [vp, vp, c_int, c_double, c_double, c_double, c_double, c_double,
                                                  c_double]),
             "hgpm_update_positions": (c_int, [vp, c_double]),
@@ -480,15 +481,27 @@ class MemoryManager:
         return v.value
 
     def autotune(self, reps=3):
-        """Times the FFT variants on this grid and keeps the fastest; returns the
-        per-variant milliseconds (column pass, fused x pass of the solve)."""
+        """Times the FFT variants on this grid and keeps the fastest, then, when the context
+        has particles, the deposit tile and gather shapes; returns the per-variant
+        milliseconds (column pass, fused x pass of the solve, deposit, gather)."""
         n = self.get_tunable("fft_variants")
         mc = np.zeros(n)
         mx = np.zeros(n)
         self._c(lib().hgpm_autotune(self.h, int(reps), _ptr(mc), _ptr(mx)))
-        return {"cols_ms": [v if v >= 0 else None for v in mc.tolist()],
-                "xsolve_ms": [v if v >= 0 else None for v in mx.tolist()],
-                "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve")}
+        out = {"cols_ms": [v if v >= 0 else None for v in mc.tolist()],
+               "xsolve_ms": [v if v >= 0 else None for v in mx.tolist()],
+               "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve")}
+        if self.precision == FP32:
+            # the particle kernels, on this context's particles when it has them
+            md, mg = np.zeros(2), np.zeros(2)
+            rc = lib().hgpm_autotune_particles(self.h, int(reps), _ptr(md), _ptr(mg))
+            if rc == OK:
+                out.update({"deposit_ms": md.tolist(), "gather_ms": mg.tolist(),
+                            "deposit_tile": self.get_tunable("deposit_tile"),
+                            "gather_tile": self.get_tunable("gather_tile")})
+            elif rc != ERR_STATE:
+                self._c(rc)
+        return out
 
     # instrumentation
     def set_timing(self, on=True):

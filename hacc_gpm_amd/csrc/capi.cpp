@@ -92,6 +92,7 @@ struct hgpm_ctx {
     // hgpm_set_tunable or hgpm_autotune
     int var_cols = 2, var_xs = 2;
     int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
+    int dep_ppt = 8;    // deposit tile: a whole 16^3 Lagrangian tile (8) or half (4), tunable "deposit_tile"
 };
 
 namespace {
@@ -242,8 +243,9 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.boxes = c->boxes;
     b.nbox = c->nbox;
     b.totals = c->total;
+    b.ppt = c->dep_ppt;
     // about 256 tiles per sub-arena, at most HGPM_DEP_NSUB sub-arenas
-    const int64_t nt = c->N * c->nbox / (16 * 16 * 16);
+    const int64_t nt = c->N * c->nbox / (512 * c->dep_ppt);
     b.nsub = (int)std::max<int64_t>(1, std::min<int64_t>(HGPM_DEP_NSUB, nt / 256));
     return b;
 }
@@ -481,7 +483,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
             // runs measured here through z = 0 (the windows' total grows from ~1.4 G to
             // ~4 G cells as structure forms); tiles beyond it take the atomic route
             const size_t nblk = (size_t)(p->ng / 16) * (p->ng / 16) * (p->ng / 16) * nbox;
-            const size_t ntiles = N * nbox / (16 * 16 * 16);
+            const size_t ntiles = N * nbox / (8 * 16 * 16); // half tiles: the smaller deposit tile
             c->arena_alloc = 6ull * G * nbox;
             c->arena_cap = c->arena_alloc;
             if ((rc = dalloc(c, &c->arena, sizeof(unsigned) * c->arena_alloc, "deposit arena")) ||
@@ -492,7 +494,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
                 (rc = dalloc(c, &c->wide_slots, sizeof(unsigned short) * HGPM_DEP_WKEEP * ntiles,
                              "deposit 64-bit slots")) ||
                 (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
-                (rc = dalloc(c, (char**)&c->boxes, 64 * ntiles, "gather boxes")))
+                (rc = dalloc(c, (char**)&c->boxes, 32 * ntiles, "gather boxes")))
                 return bail(rc);
             // particles start at cell 0, offset 0 (valid if never set; a step refuses them)
             if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess ||
@@ -1323,6 +1325,13 @@ int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
         c->var_gather = value;
         return HGPM_OK;
     }
+    if (!strcmp(key, "deposit_tile")) {
+        if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "deposit_tile needs an FP32 context");
+        if (value != 0 && value != 1) return fail(c, HGPM_ERR_ARG, "deposit_tile must be 0 (16^3) or 1 (8 x 16 x 16)");
+        c->dep_ppt = value == 0 ? 8 : 4;
+        c->boxes_valid = false;
+        return HGPM_OK;
+    }
     if (!strcmp(key, "deposit_slots")) {
         if (value < 1 || value > HGPM_DEP_SLOTS)
             return fail(c, HGPM_ERR_ARG, "deposit_slots must be in [1," + std::to_string(HGPM_DEP_SLOTS) + "]");
@@ -1343,6 +1352,7 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
     if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
+    if (!strcmp(key, "deposit_tile")) { *value = c->dep_ppt == 8 ? 0 : 1; return HGPM_OK; }
     if (!strcmp(key, "deposit_arena_mwords")) {
         // 32-bit arena words the last deposit reserved, in units of 2^20: the windows'
         // reservations, those of tiles then found full included, plus the doubled ones
@@ -1418,6 +1428,78 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     (void)hipEventDestroy(b);
     c->have_density = c->have_force = false;
     return HGPM_OK;
+}
+
+// Times the deposit tile variants (16^3 and 8 x 16 x 16 Lagrangian blocks per workgroup)
+// and the gather workgroup shapes on the context's own particles and keeps the fastest.
+// Particles are left as they were: the deposit writes only grids, and the gather runs
+// as a kick of zero length without the drift on velocities that are restored afterwards.
+// Overwrites the density and the solve's grids.
+int hgpm_autotune_particles(hgpm_ctx* c, int reps, double* ms_deposit, double* ms_gather) {
+    if (!c) return HGPM_ERR_ARG;
+    if (c->prec != HGPM_FP32) return HGPM_OK;
+    int rc = need_particles(c);
+    if (rc) return rc;
+    if (reps < 1) reps = 3;
+    const bool timing = c->timing;
+    c->timing = false;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    auto timed = [&](auto&& run, double& per) -> int {
+        int r = run();
+        if (r) return r;
+        HIPCHK(c, hipEventRecord(a, c->st));
+        for (int i = 0; i < reps; i++)
+            if ((r = run())) return r;
+        HIPCHK(c, hipEventRecord(b, c->st));
+        HIPCHK(c, hipEventSynchronize(b));
+        LAUNCHCHK(c);
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+        per = ms / reps;
+        return HGPM_OK;
+    };
+    double best = 1e30;
+    int best_ppt = c->dep_ppt;
+    const int ppts[2] = {8, 4};
+    for (int v = 0; v < 2 && !rc; v++) {
+        c->dep_ppt = ppts[v];
+        double per = 0.0;
+        rc = timed([&]() { return f32_deposit(c, false); }, per);
+        if (ms_deposit) ms_deposit[v] = per;
+        if (!rc && per < best) { best = per; best_ppt = ppts[v]; }
+    }
+    c->dep_ppt = best_ppt;
+    if (!rc) rc = f32_deposit(c, false); // the boxes of the kept tile shape
+    // the gather: velocities saved, a zero-length kick timed, velocities restored
+    float* saved = nullptr;
+    const size_t vb = sizeof(float) * 3 * (size_t)c->N * c->nbox;
+    if (!rc && dalloc(c, &saved, vb, "velocity copy") == HGPM_OK) {
+        c->bytes -= vb;
+        hipError_t e = hipMemcpyAsync(saved, c->vel, vb, hipMemcpyDeviceToDevice, c->st);
+        double bestg = 1e30;
+        for (int v = 0; v < 2 && e == hipSuccess && !rc; v++) {
+            double per = 0.0;
+            rc = timed([&]() {
+                launch_gather_kick(c->pos, c->vel, c->N * c->nbox, c->ng, c->psi, 0.0, 0.0, 0, 0.0f, v,
+                                   c->boxes_valid ? c->boxes : nullptr, c->nbox, c->st);
+                return HGPM_OK;
+            }, per);
+            if (ms_gather) ms_gather[v] = per;
+            if (!rc && per < bestg) { bestg = per; c->var_gather = v; }
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(c->vel, saved, vb, hipMemcpyDeviceToDevice, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        (void)hipFree(saved);
+        if (!rc && e != hipSuccess)
+            rc = fail(c, HGPM_ERR_HIP, std::string("hgpm_autotune_particles: ") + hipGetErrorString(e));
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->timing = timing;
+    c->have_density = c->have_force = false;
+    return rc;
 }
 
 } // extern "C"
@@ -1497,12 +1579,12 @@ struct BoxView {
 size_t batch_bytes(const hgpm_params& p, int nstore, int nwave) {
     const size_t N = (size_t)p.np * p.np * p.np, G = (size_t)p.ng * p.ng * p.ng;
     const size_t nblk = (size_t)(p.ng / 16) * (p.ng / 16) * (p.ng / 16) * nwave;
-    const size_t ntiles = N * nwave / (16 * 16 * 16);
+    const size_t ntiles = N * nwave / (8 * 16 * 16); // half tiles
     size_t b = (sizeof(PosPackH) + 3 * sizeof(float)) * N * nstore;     // particles
     b += (2 * sizeof(float2) + sizeof(float)) * G * nwave;              // A, B, psi
     b += 6 * sizeof(unsigned) * G * nwave;                              // arena
     b += (16 * HGPM_DEP_SLOTS + sizeof(unsigned)) * nblk;               // slots, counts
-    b += (2 * sizeof(unsigned) + sizeof(unsigned long long) + 64 + 2 * HGPM_DEP_WKEEP) * ntiles; // tile buffers
+    b += (2 * sizeof(unsigned) + sizeof(unsigned long long) + 32 + 2 * HGPM_DEP_WKEEP) * ntiles; // tile buffers
     b += 64ull << 20;                                                   // tables, P(k), slack
     return b;
 }
@@ -1812,6 +1894,14 @@ int hgpm_ensemble_autotune(hgpm_ensemble* e, int reps) {
     if (!e) return HGPM_ERR_ARG;
     for (hgpm_ctx* c : e->batch) {
         int rc = hgpm_autotune(c, reps, nullptr, nullptr);
+        if (rc) return eforward(e, c, rc);
+    }
+    // the particle kernels once every box has its particles (a batch stored in waves is
+    // tuned on its first wave)
+    for (char r : e->ready)
+        if (!r) return HGPM_OK;
+    for (hgpm_ctx* c : e->batch) {
+        int rc = hgpm_autotune_particles(c, reps, nullptr, nullptr);
         if (rc) return eforward(e, c, rc);
     }
     return HGPM_OK;

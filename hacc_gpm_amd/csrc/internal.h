@@ -68,6 +68,8 @@ void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf
 //              chunks); [2] tiles stored with 64-bit cells; [4 + s] cells reserved in
 //              sub-arena s;
 //   tile_total: deposited mass per tile.
+// A deposit tile is DEP_NT * ppt consecutive particles: a whole 16^3 Lagrangian tile
+// (ppt 8) or half of one (ppt 4); per-tile buffers are sized for half tiles.
 #define HGPM_DEP_SLOTS 64
 #define HGPM_DEP_WKEEP 32
 #define HGPM_DEP_NSUB 64
@@ -87,6 +89,7 @@ struct DepositBuffers {
     void* boxes; // 32 B per half tile: the cell box the gather stages (written by the windows pass)
     int nbox;    // boxes of the same shape stored back to back (particles, grids, owner blocks)
     unsigned long long* totals; // deposited mass per box
+    int ppt;                    // particles per thread of a deposit tile: 8 or 4
 };
 // The windows pass with 32-bit cells, then the 64-bit pass over the tiles it listed (a
 // cell over 2^32 - 1 mass units).  ctl must be cleared and the chunk counts zero.

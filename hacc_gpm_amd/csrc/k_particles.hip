@@ -107,7 +107,7 @@ template <bool SMALL> __device__ __forceinline__ DepW<SMALL> fixed_weight_w(floa
 //                       DEP_MAXCH chunks) are deposited afterwards with int64 atomics.
 // Integer sums are order-free, so every route gives the bits of a sequential sum.
 constexpr int DEP_NT = 512;
-constexpr int DEP_PPT = 8;
+constexpr int DEP_PPT = 8;                 // particles per thread of a whole Lagrangian tile
 constexpr int DEP_TILE = DEP_NT * DEP_PPT; // = HGPM_TL^3
 constexpr int DEP_LDS = 79872;             // window bytes per workgroup (two per CU: 160 KB)
 constexpr int OB = 16;                     // owner block edge (cells)
@@ -134,17 +134,20 @@ __device__ __forceinline__ int wrap_cell(int v, int ng) {
 
 // Registers of one tile's particles: cell relative to the first particle's cell r0
 // (periodic-unwrapped) and offsets.
-struct DepRegs {
-    int rx[DEP_PPT], ry[DEP_PPT], rz[DEP_PPT];
-    float fx[DEP_PPT], fy[DEP_PPT], fz[DEP_PPT];
+// PPT particles per thread: DEP_NT * PPT consecutive particles, a whole Lagrangian tile
+// (PPT = 8) or half of one (PPT = 4, an 8 x 16 x 16 lattice block).
+template <int PPT> struct DepRegs {
+    int rx[PPT], ry[PPT], rz[PPT];
+    float fx[PPT], fy[PPT], fz[PPT];
 };
 
 // Loads the tile starting at p0 and reduces the box of its cells: lo = minimum
 // relative cell, e = extent + 1 (the upper CIC neighbour).  The two halves of the tile
-// (particles k < DEP_PPT/2 and k >= DEP_PPT/2 of every thread: the gather's tiles) get
-// their own boxes too: hb[h] = {lo[3], base cells per axis[3]}.
+// (particles k < PPT/2 and k >= PPT/2 of every thread: the gather's tiles when PPT = 8)
+// get their own boxes too: hb[h] = {lo[3], base cells per axis[3]}.
+template <int PPT>
 __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int64_t p0, int ng,
-                                             const PosSplit& r0, DepRegs& R, int (*red)[DEP_NT / 64], int lo[3],
+                                             const PosSplit& r0, DepRegs<PPT>& R, int (*red)[DEP_NT / 64], int lo[3],
                                              int e[3], int (*hb)[6] = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     int mn[2][3], mx[2][3];
@@ -153,8 +156,8 @@ __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int6
 #pragma unroll
         for (int d = 0; d < 3; d++) { mn[h][d] = INT_MAX; mx[h][d] = INT_MIN; }
 #pragma unroll
-    for (int k = 0; k < DEP_PPT; k++) {
-        const int h = k < DEP_PPT / 2 ? 0 : 1;
+    for (int k = 0; k < PPT; k++) {
+        const int h = k < PPT / 2 ? 0 : 1;
         const int64_t j = p0 + tid + k * DEP_NT;
         const PosSplit p = unpack(pos[j]); // tiles are full: n is a multiple of DEP_TILE
         R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
@@ -210,8 +213,8 @@ template <int WM, typename F> __device__ __forceinline__ void dep_slabs(const in
 // Adds the corners of the tile's particles that fall in the slab (sx0.., nx) x (sy0.., ny)
 // x (all z) to win (slab order x, y, z); returns this thread's sum of the added weights.
 // ALL: the slab is the whole window, every corner is inside (no checks).
-template <bool SMALL, bool ALL, typename W>
-__device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRegs& R, const int lo[3],
+template <bool SMALL, bool ALL, int PPT, typename W>
+__device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRegs<PPT>& R, const int lo[3],
                                                              const int e[3], int sx0, int nx, int sy0, int ny,
                                                              float mass) {
     unsigned long long dep = 0;
@@ -219,7 +222,7 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
     const int lx = (int)(threadIdx.x >> 1) & 1, ly = (int)threadIdx.x & 1;
     const int ox_[2] = {lx * sxs, (1 - lx) * sxs}, oy_[2] = {ly * ez, (1 - ly) * ez};
 #pragma unroll
-    for (int k = 0; k < DEP_PPT; k++) {
+    for (int k = 0; k < PPT; k++) {
         // the weights are slab-invariant; an opaque copy of the offsets stops the
         // compiler from hoisting all 64 of them out of the slab loop (register spill)
         float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
@@ -269,8 +272,8 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
 template <bool SMALL> __device__ __forceinline__ bool dep_one_pass(const int e[3]) {
     return e[0] * e[1] * e[2] <= DEP_LDS / (int)sizeof(DepW<SMALL>);
 }
-template <bool SMALL>
-__device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
+template <bool SMALL, int PPT>
+__device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRegs<PPT>& R, const int lo[3],
                                                    const int e[3], float mass, unsigned long long& dep) {
     const int vol = e[0] * e[1] * e[2];
     if constexpr (SMALL) {
@@ -322,8 +325,8 @@ __device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int 
 }
 // WIDE (64-bit LDS window, !SMALL): every cell is stored whole as two 32-bit arena
 // words, cell i of the window at words base + 2i, base + 2i + 1.
-template <bool SMALL, bool WIDE = false>
-__device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const DepRegs& R, const int lo[3],
+template <bool SMALL, bool WIDE, int PPT>
+__device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const DepRegs<PPT>& R, const int lo[3],
                                                        const int e[3], float mass, unsigned* __restrict arena,
                                                        unsigned long long base, unsigned long long& dep,
                                                        unsigned long long& fl) {
@@ -370,8 +373,8 @@ __device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const D
 
 // Atomic route (deferred tiles): int64 window slab by slab, non-zero cells added to the
 // density grid rho with int64 atomics.  Returns this thread's share of the deposited mass.
-template <bool SMALL>
-__device__ __forceinline__ long long dep_window_atomic(unsigned long long* win, const DepRegs& R, const int lo[3],
+template <bool SMALL, int PPT>
+__device__ __forceinline__ long long dep_window_atomic(unsigned long long* win, const DepRegs<PPT>& R, const int lo[3],
                                                        const int e[3], const PosSplit& r0, int ng, float mass,
                                                        unsigned long long* __restrict rho) {
     constexpr int WM = DEP_LDS / 8;
@@ -486,7 +489,7 @@ struct DepWindowShared {
 // Only the 32-bit pass stores the gather's boxes.  q: the tile's entry of wide (WIDE);
 // a tile of at most DEP_WKEEP chunks takes the slots it had in the 32-bit pass again
 // (the same chunks, in the same order), a larger one lists its chunks anew.
-template <bool SMALL, bool WIDE>
+template <bool SMALL, bool WIDE, int PPT>
 __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const DepWindowArgs& g,
                                                  DepWindowShared& sh) {
     constexpr bool S32 = SMALL && !WIDE;           // 32-bit LDS window
@@ -494,29 +497,40 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
     DepW<S32>* win = (DepW<S32>*)sh.win;
     const int tid = threadIdx.x;
     const int ng = g.ng;
-    const int64_t p0 = (int64_t)tile * DEP_TILE;
+    const int64_t p0 = (int64_t)tile * (DEP_NT * PPT);
     const PosSplit r0 = unpack(g.pos[p0]);
     if (tid == 0) { sh.over = 0; sh.full = 0; } // ordered by the barrier in dep_load_box
-    DepRegs R;
+    DepRegs<PPT> R;
     int lo[3], e[3], hb[2][6];
     dep_load_box(g.pos, p0, ng, r0, R, sh.red, lo, e, hb);
     if (!WIDE && tid == 0) {
-        // the gather of these positions stages its potential window from these boxes.
-        // Their origins are stored as absolute cells (centred into [-ng/2, ng/2)): the
-        // gather must not re-read r0, whose particle another gather workgroup drifts.
+        // the gather of these positions stages its potential window from these boxes (one
+        // per half Lagrangian tile, the gather's workgroup: both halves of a whole tile,
+        // or the whole box of a half tile).  Their origins are stored as absolute cells
+        // (centred into [-ng/2, ng/2)): the gather must not re-read r0, whose particle
+        // another gather workgroup drifts.
+        if constexpr (PPT == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                hb[0][d] = lo[d];
+                hb[0][3 + d] = e[d] - 1;
+            }
+        }
+        constexpr int NH = PPT == 8 ? 2 : 1;
         int o[2][3];
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < NH; h++)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 const int rc = d == 0 ? r0.cx : (d == 1 ? r0.cy : r0.cz);
                 int a = wrap_cell(rc + hb[h][d], ng);
                 o[h][d] = a >= ng / 2 ? a - ng : a;
             }
-        g.boxes[4 * tile + 0] = make_int4(o[0][0], o[0][1], o[0][2], hb[0][3]);
-        g.boxes[4 * tile + 1] = make_int4(hb[0][4], hb[0][5], 0, 0);
-        g.boxes[4 * tile + 2] = make_int4(o[1][0], o[1][1], o[1][2], hb[1][3]);
-        g.boxes[4 * tile + 3] = make_int4(hb[1][4], hb[1][5], 0, 0);
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            g.boxes[2 * (NH * tile + h) + 0] = make_int4(o[h][0], o[h][1], o[h][2], hb[h][3]);
+            g.boxes[2 * (NH * tile + h) + 1] = make_int4(hb[h][4], hb[h][5], 0, 0);
+        }
     }
     const int nb = ng / OB;
     const int A[3] = {r0.cx + lo[0], r0.cy + lo[1], r0.cz + lo[2]};
@@ -621,19 +635,19 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
     }
 }
 
-template <bool SMALL>
+template <bool SMALL, int PPT>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(DepWindowArgs g) {
     __shared__ __align__(16) DepWindowShared sh;
-    dep_windows_tile<SMALL, false>((int)xcd_remap(blockIdx.x, gridDim.x), 0u, g, sh);
+    dep_windows_tile<SMALL, false, PPT>((int)xcd_remap(blockIdx.x, gridDim.x), 0u, g, sh);
 }
 
 // The tiles the 32-bit pass listed in wide (ctl[2] of them), before the owner pass.
-template <bool SMALL>
+template <bool SMALL, int PPT>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_wide(DepWindowArgs g) {
     __shared__ __align__(16) DepWindowShared sh;
     const unsigned long long nw = g.ctl[2];
     for (unsigned long long q = blockIdx.x; q < nw; q += gridDim.x) {
-        dep_windows_tile<SMALL, true>((int)g.wide[q], (unsigned)q, g, sh);
+        dep_windows_tile<SMALL, true, PPT>((int)g.wide[q], (unsigned)q, g, sh);
         __syncthreads();
     }
 }
@@ -756,7 +770,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
 }
 
 // Tiles listed by k_deposit_windows as deferred, after k_deposit_owner: int64 atomics.
-template <bool SMALL>
+template <bool SMALL, int PPT>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosPack* __restrict pos, int64_t n, int ng,
                                                              float mass, unsigned long long* __restrict rho,
                                                              const unsigned long long* __restrict ctl,
@@ -769,9 +783,9 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_deferred(const PosPack* __re
     const unsigned long long nd = ctl[1];
     for (unsigned long long q = blockIdx.x; q < nd; q += gridDim.x) {
         const unsigned tile = deferred[q];
-        const int64_t p0 = (int64_t)tile * DEP_TILE;
+        const int64_t p0 = (int64_t)tile * (DEP_NT * PPT);
         const PosSplit r0 = unpack(pos[p0]);
-        DepRegs R;
+        DepRegs<PPT> R;
         int lo[3], e[3];
         dep_load_box(pos, p0, ng, r0, R, red, lo, e);
         unsigned long long* box_rho = rho + (int64_t)(tile / ntb) * gcells;
@@ -1319,21 +1333,30 @@ DepWindowArgs window_args(const PosPackH* pos, int64_t n, int ng, float mass, co
     g.tile_total = b.tile_total;
     g.nsub = b.nsub;
     g.boxes = (int4*)b.boxes;
-    g.ntb = (int)(n / b.nbox / DEP_TILE);
+    g.ntb = (int)(n / b.nbox / (DEP_NT * b.ppt));
     g.nblk = (ng / OB) * (ng / OB) * (ng / OB);
     return g;
 }
 void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s) {
     if (!n) return;
-    const unsigned nt = (unsigned)(n / DEP_TILE);
+    const unsigned nt = (unsigned)(n / (DEP_NT * b.ppt));
     const DepWindowArgs g = window_args(pos, n, ng, mass, b);
-    if (mass < 128.0f) k_deposit_windows<true><<<nt, DEP_NT, 0, s>>>(g);
-    else k_deposit_windows<false><<<nt, DEP_NT, 0, s>>>(g);
-    // the tiles with a cell over 32 bits (none, mostly: the workgroups return at once)
+    // the tiles with a cell over 32 bits (none, mostly: the workgroups of the second
+    // launch return at once)
     const unsigned grid = nt < 256 ? nt : 256;
-    if (mass < 128.0f) k_deposit_wide<true><<<grid, DEP_NT, 0, s>>>(g);
-    else k_deposit_wide<false><<<grid, DEP_NT, 0, s>>>(g);
+    auto run = [&](auto pc) {
+        constexpr int PPT = decltype(pc)::value;
+        if (mass < 128.0f) {
+            k_deposit_windows<true, PPT><<<nt, DEP_NT, 0, s>>>(g);
+            k_deposit_wide<true, PPT><<<grid, DEP_NT, 0, s>>>(g);
+        } else {
+            k_deposit_windows<false, PPT><<<nt, DEP_NT, 0, s>>>(g);
+            k_deposit_wide<false, PPT><<<grid, DEP_NT, 0, s>>>(g);
+        }
+    };
+    if (b.ppt == 4) run(std::integral_constant<int, 4>{});
+    else run(std::integral_constant<int, 8>{});
 }
 void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
@@ -1344,24 +1367,30 @@ void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuff
 }
 void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
-    const int ntb = (int)(n / DEP_TILE / b.nbox);
+    const int ntb = (int)(n / (DEP_NT * b.ppt) / b.nbox);
     k_zero_u64<<<1, 64, 0, s>>>(b.totals, (size_t)b.nbox);
     k_deposit_total<<<b.nbox * DEP_TOT_WG, 256, 0, s>>>(b.tile_total, ntb, b.totals);
 }
 void launch_deposit_deferred(const PosPackH* pos, int64_t n, int ng, float mass, long long* rho,
                              const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
-    const int64_t nt = n / DEP_TILE;
+    const int64_t nt = n / (DEP_NT * b.ppt);
     const int ntb = (int)(nt / b.nbox);
     const int64_t gc = (int64_t)ng * ng * ng;
     const unsigned grid = (unsigned)(nt < 512 ? nt : 512);
-    if (mass < 128.0f)
-        k_deposit_deferred<true><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass, (unsigned long long*)rho,
-                                                         b.ctl, b.deferred, b.tile_total, ntb, gc);
-    else
-        k_deposit_deferred<false><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass,
-                                                          (unsigned long long*)rho, b.ctl, b.deferred, b.tile_total,
-                                                          ntb, gc);
+    auto run = [&](auto pc) {
+        constexpr int PPT = decltype(pc)::value;
+        if (mass < 128.0f)
+            k_deposit_deferred<true, PPT><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass,
+                                                                  (unsigned long long*)rho, b.ctl, b.deferred,
+                                                                  b.tile_total, ntb, gc);
+        else
+            k_deposit_deferred<false, PPT><<<grid, DEP_NT, 0, s>>>((const PosPack*)pos, n, ng, mass,
+                                                                   (unsigned long long*)rho, b.ctl, b.deferred,
+                                                                   b.tile_total, ntb, gc);
+    };
+    if (b.ppt == 4) run(std::integral_constant<int, 4>{});
+    else run(std::integral_constant<int, 8>{});
     launch_deposit_totals(n, b, s);
 }
 template <int NT>

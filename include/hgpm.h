@@ -302,6 +302,14 @@ int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
    for variants not available on this grid; may be NULL).
    Clobbers the density and the solve's grids. */
 int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
+/* Autotuner of the particle kernels (no reference counterpart), on the context's own
+   particles (HGPM_ERR_STATE without them): times the deposit tile shapes (tunable
+   "deposit_tile": 0 = a 16^3 Lagrangian block per workgroup, 1 = an 8 x 16 x 16 block)
+   and the gather workgroup shapes ("gather_tile" 0 and 1) and keeps the fastest of
+   each; ms_deposit and ms_gather (may be NULL) receive 2 entries each.  The particles
+   are left unchanged (the gather is timed as a zero-length kick without drift on
+   velocities restored afterwards); the density and the solve's grids are overwritten. */
+int hgpm_autotune_particles(hgpm_ctx* ctx, int reps, double* ms_deposit, double* ms_gather);
 
 /* ------------------------------------------------------------- ensembles ---- */
 /* No reference counterpart (HACC_GPM runs one box per process): many boxes stepped
@@ -342,7 +350,8 @@ int hgpm_ensemble_step(hgpm_ensemble* e, const hgpm_step_coeffs* coeffs);
    timing (hgpm_set_timing) does not see the launches inside the graph. */
 int hgpm_ensemble_run(hgpm_ensemble* e, int nsteps, const hgpm_step_coeffs* coeffs);
 int hgpm_ensemble_synchronize(hgpm_ensemble* e);
-/* hgpm_autotune on every batch. */
+/* hgpm_autotune on every batch, then hgpm_autotune_particles on every batch once every
+   box has its particles (a batch stored in waves is tuned on its first wave). */
 int hgpm_ensemble_autotune(hgpm_ensemble* e, int reps);
 
 #ifdef __cplusplus

@@ -143,6 +143,69 @@ def test_deposit_mass_ratio(hg, orc):
         assert np.array_equal(m.get_density(), orc.cic_split(64, cell, off, 8.0))
 
 
+@pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64), (256, 256)])
+def test_deposit_half_tiles(hg, orc, ng, np_):
+    """The deposit with half Lagrangian tiles per workgroup (tunable deposit_tile 1: 8 x
+    16 x 16 lattice blocks, twice the tiles): bit-exact against the oracle on the arena
+    and on the atomic route, and a step with it (the gather staging from the half tiles'
+    boxes) bit-identical to a step with whole tiles and to the separate calls."""
+    case(f"deposit half tiles ng={ng} np={np_}")
+    rng = np.random.default_rng(ng * 3 + np_)
+    mass = (ng / np_) ** 3
+    p = hg.make_params(ng, np_, 2.0 * ng, 50.0, 0.0, 100)
+    with hg.MemoryManager(p) as m, hg.MemoryManager(p) as m2, hg.MemoryManager(p) as m3:
+        cell, off = lattice_split(rng, np_, ng, 1.5, outliers=3)
+        vel = rng.normal(0, 0.7, (m.N, 3)).astype(np.float32)
+        ref = orc.cic_split(ng, cell, off, mass)
+        m.set_tunable("deposit_tile", 1)
+        assert m.get_tunable("deposit_tile") == 1
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_deferred") < m.N // 2048
+        m.set_tunable("deposit_arena", 0)
+        m.cic()
+        assert np.array_equal(m.get_density(), ref)
+        assert m.get_tunable("deposit_deferred") == m.N // 2048
+        m.set_tunable("deposit_arena", -1)
+        m2.set_particles_split(cell, off, vel)
+        m3.set_tunable("deposit_tile", 1)
+        m3.set_particles_split(cell, off, vel)
+        for co in hg.schedule(p, 3, merge=True):
+            m.step(co)
+            m2.step(co)
+            if not co.skip_drift0:
+                m3.update_positions(co.drift0)
+            m3.cic()
+            m3.solve_gradient()
+            m3.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
+            a, b, c = m.get_particles_split(), m2.get_particles_split(), m3.get_particles_split()
+            for x, y, z in zip(a, b, c):
+                assert np.array_equal(x, y) and np.array_equal(x, z)
+
+
+def test_autotune_particles_leaves_particles(hg):
+    """hgpm_autotune_particles times the deposit tile and gather shapes on the context's
+    particles and keeps valid choices; positions and velocities are unchanged."""
+    case("autotune particles 64^3")
+    p = hg.make_params(64, 64, 128.0, 50.0, 0.0, 100, seed=9)
+    with hg.MemoryManager(p) as m:
+        with pytest.raises(hg.HgpmError) as e:  # no particles yet: refused
+            m._c(hg.lib().hgpm_autotune_particles(m.h, 1, None, None))
+        assert e.value.code == hg.ERR_STATE
+        rng = np.random.default_rng(9)
+        cell, off = lattice_split(rng, 64, 64, 2.0)
+        vel = rng.normal(0, 1, (m.N, 3)).astype(np.float32)
+        vel[:10] = -0.0
+        m.set_particles_split(cell, off, vel)
+        t = m.autotune(2)
+        assert t["deposit_tile"] in (0, 1) and t["gather_tile"] in (0, 1)
+        assert all(x > 0 for x in t["deposit_ms"] + t["gather_ms"])
+        c2, o2, v2 = m.get_particles_split()
+        assert np.array_equal(c2, cell) and np.array_equal(o2, off)
+        assert np.array_equal(v2.view(np.uint32), vel.view(np.uint32))
+
+
 @pytest.mark.parametrize("ng,np_,sigma", [(192, 32, 1.5), (192, 32, 0.2), (256, 32, 1.5)])
 def test_deposit_heavy_particles(hg, orc, ng, np_, sigma):
     """Particles of mass (ng/np)^3 >= 128 (216 at 192/32, 512 at 256/32) take the
