@@ -48,10 +48,10 @@ def kernel_bytes(name, Np, G):
     """Algorithmic bytes per launch of each timed kernel region: what the kernel must
     read and write once (no re-reads, no atomics beyond one write per cell)."""
     table = {
-        "drift": 44 * Np,                      # read pos 16 + vel 12, write pos 16
+        "drift": 36 * Np,                      # read pos 12 + vel 12, write pos 12
         "zero_density": 8 * G,                 # write int64 density
-        "deposit": 16 * Np + 4 * G,            # read pos (16 B packed), write the FP32 density of the step
-        "deposit_windows": 16 * Np,            # read pos (the window arena is overhead)
+        "deposit": 12 * Np + 4 * G,            # read pos (12 B fixed point), write the FP32 density of the step
+        "deposit_windows": 12 * Np,            # read pos (the window arena is overhead)
         "deposit_owner": 4 * G,                # write the FP32 density (the step path; int64 for hgpm_cic)
         # half spectra (k_z = 0..N/2) cost 4 B per grid cell
         "fft_fwd_z_r2c": 8 * G,                # read the FP32 density, write the half spectrum
@@ -59,8 +59,8 @@ def kernel_bytes(name, Np, G):
         "fft_x_psi": 8 * G,                    # read rho^, write psi^
         "fft_inv_y": 8 * G,                    # read + write one half spectrum
         "fft_inv_z_c2r": 8 * G,                # read psi^, write psi (FP32)
-        "gather_kick_drift": 56 * Np + 4 * G,  # pos (16 B) + vel (12 B) read and write, potential read
-        "gather_kick": 44 * Np + 4 * G,        # pos + vel read, vel write, potential read
+        "gather_kick_drift": 48 * Np + 4 * G,  # pos (12 B) + vel (12 B) read and write, potential read
+        "gather_kick": 36 * Np + 4 * G,        # pos + vel read, vel write, potential read
     }
     return table.get(name)
 
@@ -522,14 +522,14 @@ def main():
     breakdown = {k: {"ms_per_step": v[0] / args.steps, "launches": v[1],
                      "GBps": (kernel_bytes(k, Np, G) / (v[0] / v[1] * 1e-3) / 1e9) if kernel_bytes(k, Np, G) and v[1] else None}
                  for k, v in sorted(tim.items(), key=lambda kv: -kv[1][0])}
-    # algorithmic bytes of one step of this algorithm: 72 per particle (deposit read of
-    # the 16-byte packed position, gather/kick/drift read and write of position and
-    # velocity 56) and 48 per grid cell (FP32 density write 4, the five half-spectrum
+    # algorithmic bytes of one step of this algorithm: 60 per particle (deposit read of
+    # the 12-byte fixed-point position, gather/kick/drift read and write of position and
+    # velocity 48) and 48 per grid cell (FP32 density write 4, the five half-spectrum
     # passes 8 each, potential read 4).  With 24-byte positions, an int64 density and a
     # solve that transforms the three force fields a step moves 96 Np + 120 G with half
     # spectra and 96 Np + 200 G with full complex spectra: the same step's rate against
     # those byte counts is reported beside it.
-    step_alg = 72 * Np + 48 * G
+    step_alg = 60 * Np + 48 * G
     step_r2c = 96 * Np + 120 * G
     step_c2c = 96 * Np + 200 * G
     out = {

@@ -692,9 +692,16 @@ int hgpm_set_particles_split(hgpm_ctx* c, const int32_t* cell, const float* off,
             if (!std::isfinite(vel[3 * i + d]))
                 return fail(c, HGPM_ERR_ARG, "non-finite velocity at index " + std::to_string(i));
         }
-        const unsigned pc = ((unsigned)cell[3 * i] << (2 * HGPM_CELL_BITS_H)) |
-                            ((unsigned)cell[3 * i + 1] << HGPM_CELL_BITS_H) | (unsigned)cell[3 * i + 2];
-        h[i] = PosPackH{pc, off[3 * i], off[3 * i + 1], off[3 * i + 2]};
+        // (cell + offset) to the nearest 2^-22 cell, wrapped: an offset within 2^-23 of
+        // 1 carries into the next cell
+        unsigned X[3];
+        for (int d = 0; d < 3; d++) {
+            const long long span = (long long)c->ng << HGPM_FRAC_BITS_H;
+            long long v = ((long long)cell[3 * i + d] << HGPM_FRAC_BITS_H) +
+                          (long long)std::rint((double)off[3 * i + d] * (double)(1u << HGPM_FRAC_BITS_H));
+            X[d] = (unsigned)(v >= span ? v - span : v);
+        }
+        h[i] = PosPackH{X[0], X[1], X[2]};
     }
     PosPackH* sp = nullptr;
     float* sv = nullptr;
@@ -737,11 +744,12 @@ int hgpm_get_particles_split(hgpm_ctx* c, int32_t* cell, float* off, float* vel)
     (void)hipFree(sp);
     (void)hipFree(sv);
     for (size_t i = 0; i < n; i++) {
-        constexpr unsigned M = (1u << HGPM_CELL_BITS_H) - 1u;
-        cell[3 * i] = (int32_t)(h[i].cell >> (2 * HGPM_CELL_BITS_H));
-        cell[3 * i + 1] = (int32_t)((h[i].cell >> HGPM_CELL_BITS_H) & M);
-        cell[3 * i + 2] = (int32_t)(h[i].cell & M);
-        off[3 * i] = h[i].fx; off[3 * i + 1] = h[i].fy; off[3 * i + 2] = h[i].fz;
+        constexpr unsigned M = (1u << HGPM_FRAC_BITS_H) - 1u;
+        const unsigned X[3] = {h[i].x, h[i].y, h[i].z};
+        for (int d = 0; d < 3; d++) {
+            cell[3 * i + d] = (int32_t)(X[d] >> HGPM_FRAC_BITS_H);
+            off[3 * i + d] = (float)(X[d] & M) * (1.0f / (float)(1u << HGPM_FRAC_BITS_H));
+        }
     }
     return HGPM_OK;
 }

@@ -78,31 +78,49 @@ struct PosSplit {
     int cx, cy, cz;
     float fx, fy, fz;
 };
-// ... in memory, 16 B (one vector load or store): the cell as one 32-bit index with
-// HGPM_CELL_BITS per axis, x in the high bits (grids of up to 2^HGPM_CELL_BITS cells per
-// side), then the offsets.
+// ... in memory, 12 B (one 3-dword load or store): per axis one unsigned fixed-point
+// coordinate in units of 2^-HGPM_FRAC_BITS cells, X = cell * 2^22 + fraction.  The cell
+// is X >> 22 and the offset (X & (2^22 - 1)) * 2^-22, exact in FP32.  Grids of up to
+// 2^HGPM_CELL_BITS cells per side: X < ng * 2^22 <= 2^32.
 #define HGPM_CELL_BITS 10
-struct __align__(16) PosPack {
-    unsigned cell;
-    float fx, fy, fz;
+#define HGPM_FRAC_BITS 22
+struct PosPack {
+    unsigned x, y, z;
 };
-__host__ __device__ __forceinline__ unsigned pack_cell(int x, int y, int z) {
-    return ((unsigned)x << (2 * HGPM_CELL_BITS)) | ((unsigned)y << HGPM_CELL_BITS) | (unsigned)z;
+constexpr unsigned HGPM_FRAC_MASK = (1u << HGPM_FRAC_BITS) - 1u;
+__host__ __device__ __forceinline__ int fx_cell(unsigned X) { return (int)(X >> HGPM_FRAC_BITS); }
+__host__ __device__ __forceinline__ float fx_off(unsigned X) {
+    return (float)(X & HGPM_FRAC_MASK) * (1.0f / (float)(1u << HGPM_FRAC_BITS));
 }
 __host__ __device__ __forceinline__ PosSplit unpack(const PosPack& q) {
-    constexpr unsigned M = (1u << HGPM_CELL_BITS) - 1u;
     PosSplit p;
-    p.cx = (int)(q.cell >> (2 * HGPM_CELL_BITS));
-    p.cy = (int)((q.cell >> HGPM_CELL_BITS) & M);
-    p.cz = (int)(q.cell & M);
-    p.fx = q.fx; p.fy = q.fy; p.fz = q.fz;
+    p.cx = fx_cell(q.x); p.cy = fx_cell(q.y); p.cz = fx_cell(q.z);
+    p.fx = fx_off(q.x); p.fy = fx_off(q.y); p.fz = fx_off(q.z);
     return p;
 }
-__host__ __device__ __forceinline__ PosPack pack(const PosSplit& p) {
-    PosPack q;
-    q.cell = pack_cell(p.cx, p.cy, p.cz);
-    q.fx = p.fx; q.fy = p.fy; q.fz = p.fz;
-    return q;
+// Fixed-point coordinate of the grid-unit position x (FP64): rint(x 2^22) mod ng 2^22.
+__host__ __device__ __forceinline__ unsigned fx_from_double(double x, int ng) {
+    const long long span = (long long)ng << HGPM_FRAC_BITS;
+    long long s = (long long)rint(x * (double)(1u << HGPM_FRAC_BITS)) % span;
+    if (s < 0) s += span;
+    return (unsigned)s;
+}
+// Drift of one coordinate (UpdatePosKernel, update_positions_pm.cu:3-20, in fixed point):
+// the FP32 displacement t = v * pf, split into its integer part ti = trunc(t) and the
+// fraction t - ti (both exact), is added as ti 2^22 + rint((t - ti) 2^22) (ties to even)
+// and the sum wrapped modulo ng 2^22: integer steps, no rounding depends on the
+// position.  A move of a box or more (never seen in a run) is first reduced modulo ng
+// in FP32.
+__host__ __device__ __forceinline__ unsigned fx_drift(unsigned X, float v, float pf, int ng) {
+    float t = v * pf;
+    if (!(fabsf(t) < (float)ng)) t = t - (float)ng * truncf(t / (float)ng);
+    const float ti = truncf(t);
+    const long long span = (long long)ng << HGPM_FRAC_BITS;
+    long long s = (long long)X + ((long long)(int)ti << HGPM_FRAC_BITS) +
+                  (long long)(int)rintf((t - ti) * (float)(1u << HGPM_FRAC_BITS));
+    if (s < 0) s += span;
+    else if (s >= span) s -= span;
+    return (unsigned)s;
 }
 
 #define HGPM_WAVE 64

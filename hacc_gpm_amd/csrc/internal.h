@@ -49,10 +49,11 @@ inline const FftOps* fft_ops(int n) {
 }
 
 // ---- FP32 particle kernels (k_particles.hip) ----
-// Host view of a device position (PosPack in common.h): the cell as one index with
-// HGPM_CELL_BITS bits per axis (x high), then the FP32 offsets.
-struct PosPackH { unsigned cell; float fx, fy, fz; };
+// Host view of a device position (PosPack in common.h): per axis the fixed-point
+// coordinate cell * 2^22 + fraction, in units of 2^-22 cells.
+struct PosPackH { unsigned x, y, z; };
 constexpr int HGPM_CELL_BITS_H = 10;
+constexpr int HGPM_FRAC_BITS_H = 22;
 constexpr int HGPM_MAX_CELLS = 1 << HGPM_CELL_BITS_H; // cells per side of the FP32 engine's grid
 void launch_zero_i64(long long* p, size_t n, hipStream_t s);
 void launch_zero_u64(unsigned long long* p, size_t n, hipStream_t s);

@@ -193,7 +193,8 @@ __global__ void k_disp_f64(const double2* __restrict amp, double2* __restrict ou
     out[i] = make_double2(ci * kd, -cr * kd);
 }
 
-// placeParticles for one axis: x = q ng/np + delta psi (FP64, then split), v = velMul psi.
+// placeParticles for one axis: x = q ng/np + delta psi (FP64, then fixed point:
+// rint(x 2^22) mod ng 2^22), v = velMul psi.
 __global__ void k_place_split(const float2* __restrict sg, PosPack* __restrict pos, float* __restrict vel, int np,
                               int ng, int dim, double delta, float vel_mul) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; // internal (tile) order
@@ -204,18 +205,11 @@ __global__ void k_place_split(const float2* __restrict sg, PosPack* __restrict p
     const int q = dim == 0 ? qx : (dim == 1 ? qy : qz);
     float s = (float)((double)sg[i].x * (1.0 / (double)G));
     double x = (double)q * ((double)ng / (double)np) + delta * (double)s;
-    double fl = floor(x);
-    float f = (float)(x - fl);
-    int c = (int)fl;
-    if (f >= 1.0f) { f = 0.0f; c += 1; }
-    c %= ng;
-    if (c < 0) c += ng;
+    const unsigned X = fx_from_double(x, ng);
     // one axis per call: the others keep what the earlier calls placed (x, y, z order)
-    PosSplit p = dim == 0 ? PosSplit{0, 0, 0, 0.0f, 0.0f, 0.0f} : unpack(pos[j]);
-    if (dim == 0) { p.cx = c; p.fx = f; }
-    else if (dim == 1) { p.cy = c; p.fy = f; }
-    else { p.cz = c; p.fz = f; }
-    pos[j] = pack(p);
+    if (dim == 0) pos[j].x = X;
+    else if (dim == 1) pos[j].y = X;
+    else pos[j].z = X;
     vel[3 * j + dim] = vel_mul * s;
 }
 __global__ void k_place_f64(const double2* __restrict sg, double* __restrict pos, double* __restrict vel, int np,

@@ -28,41 +28,15 @@ __global__ void k_zero_u64(unsigned long long* __restrict p, size_t n) {
     for (; i < n; i += st) p[i] = 0ull;
 }
 
-// Carry the integer part of an updated offset into the cell and wrap periodically:
-// c <- (c + floor(f) [+1 if f - floor(f) rounds to 1]) mod ng.  A move of less than one
-// box needs one compare-and-add; the integer division runs only beyond that.
-__device__ __forceinline__ void carry_wrap(int& c, float& f, int ng) {
-    float fl = floorf(f);
-    float r = f - fl;
-    int cc = c + (int)fl;
-    if (r >= 1.0f) { r = 0.0f; cc += 1; }
-    if (cc >= ng) cc -= ng;
-    else if (cc < 0) cc += ng;
-    if ((unsigned)cc >= (unsigned)ng) {
-        cc %= ng;
-        if (cc < 0) cc += ng;
-    }
-    c = cc;
-    f = r;
-}
-
-__device__ __forceinline__ void drift_one(PosSplit& p, float vx, float vy, float vz, float pf, int ng) {
-    float t;
-    t = vx * pf; p.fx = p.fx + t;
-    t = vy * pf; p.fy = p.fy + t;
-    t = vz * pf; p.fz = p.fz + t;
-    carry_wrap(p.cx, p.fx, ng);
-    carry_wrap(p.cy, p.fy, ng);
-    carry_wrap(p.cz, p.fz, ng);
-}
-
 __global__ __launch_bounds__(BS) void k_drift(PosPack* __restrict pos, const float* __restrict vel,
                                               int64_t n, int ng, float pf) {
     int64_t i = xcd_remap(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if (i >= n) return;
-    PosSplit p = unpack(pos[i]);
-    drift_one(p, vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], pf, ng);
-    pos[i] = pack(p);
+    PosPack q = pos[i];
+    q.x = fx_drift(q.x, vel[3 * i], pf, ng);
+    q.y = fx_drift(q.y, vel[3 * i + 1], pf, ng);
+    q.z = fx_drift(q.z, vel[3 * i + 2], pf, ng);
+    pos[i] = q;
 }
 
 // CIC weight of the corner (x,y,z) in FP32: ((dx*dy)*dz)*mass with dx = f or 1-f,
@@ -868,7 +842,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     // particle loads are in flight.  Otherwise the box is reduced relative to this
     // workgroup's own first particle (read before this workgroup writes it).
     const PosSplit r0 = BOXED ? PosSplit{0, 0, 0, 0.0f, 0.0f, 0.0f} : unpack(pos[p0]);
-    PosSplit P[GT_PPT];
+    PosPack P[GT_PPT]; // fixed-point coordinates: cells and offsets are formed where used
     float V[GT_PPT][3];
     int lo[3], b[3];
     if constexpr (BOXED) {
@@ -879,7 +853,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
 #pragma unroll
         for (int k = 0; k < GT_PPT; k++) {
             const int64_t j = p0 + tid + k * GT_NT;
-            P[k] = unpack(pos[j]); // tiles are full: n is a multiple of GT_TILE
+            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
             V[k][0] = vel[3 * j];
             V[k][1] = vel[3 * j + 1];
             V[k][2] = vel[3 * j + 2];
@@ -889,12 +863,12 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
 #pragma unroll
         for (int k = 0; k < GT_PPT; k++) {
             const int64_t j = p0 + tid + k * GT_NT;
-            P[k] = unpack(pos[j]); // tiles are full: n is a multiple of GT_TILE
+            P[k] = pos[j]; // tiles are full: n is a multiple of GT_TILE
             V[k][0] = vel[3 * j];
             V[k][1] = vel[3 * j + 1];
             V[k][2] = vel[3 * j + 2];
-            const int ax = wrap_rel(P[k].cx - r0.cx, ng), ay = wrap_rel(P[k].cy - r0.cy, ng),
-                      az = wrap_rel(P[k].cz - r0.cz, ng);
+            const int ax = wrap_rel(fx_cell(P[k].x) - r0.cx, ng), ay = wrap_rel(fx_cell(P[k].y) - r0.cy, ng),
+                      az = wrap_rel(fx_cell(P[k].z) - r0.cz, ng);
             mn[0] = min(mn[0], ax); mx[0] = max(mx[0], ax);
             mn[1] = min(mn[1], ay); mx[1] = max(mx[1], ay);
             mn[2] = min(mn[2], az); mx[2] = max(mx[2], az);
@@ -947,7 +921,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 __syncthreads();
 #pragma unroll
                 for (int k = 0; k < GT_PPT; k++) {
-                    int lx = P[k].cx - gxs, ly = P[k].cy - gys;
+                    int lx = fx_cell(P[k].x) - gxs, ly = fx_cell(P[k].y) - gys;
                     lx += (lx >> 31) & ng;
                     ly += (ly >> 31) & ng;
                     if ((unsigned)lx < (unsigned)nbx && (unsigned)ly < (unsigned)nby) atomicOr(&bm[lx], 1ull << ly);
@@ -1040,8 +1014,9 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 const int64_t j = p0 + tid + k * GT_NT;
                 // an opaque copy keeps the slab-invariant weights and indices of all
                 // GT_PPT particles from being hoisted out of the slab loops (registers)
-                PosSplit p = P[k];
-                asm volatile("" : "+v"(p.cx), "+v"(p.cy), "+v"(p.cz), "+v"(p.fx), "+v"(p.fy), "+v"(p.fz));
+                PosPack X = P[k];
+                asm volatile("" : "+v"(X.x), "+v"(X.y), "+v"(X.z));
+                const PosSplit p = unpack(X);
                 // base cell relative to the slab's first base cell, modulo ng: the true
                 // offset lies in [0, nbx) for the slab's particles, and every other
                 // particle of the tile lands at >= nbx (its offset is in [-sx0, b - sx0)
@@ -1093,8 +1068,10 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 vel[3 * j + 1] = vy;
                 vel[3 * j + 2] = vz;
                 if (do_drift) {
-                    drift_one(p, vx, vy, vz, pf, ng);
-                    pos[j] = pack(p);
+                    X.x = fx_drift(X.x, vx, pf, ng);
+                    X.y = fx_drift(X.y, vy, pf, ng);
+                    X.z = fx_drift(X.z, vz, pf, ng);
+                    pos[j] = X;
                 }
             }
             __syncthreads();
@@ -1102,21 +1079,19 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     }
 }
 
-// float3 grid-unit positions in reference order -> (cell, offset) in internal order:
-// c = floor(x), f = x - c exact in FP32.
+// float3 grid-unit positions in reference order -> fixed point in internal order:
+// rint(x 2^22) mod ng 2^22 (x 2^22 is exact in FP64).
 __global__ void k_split_from_float3(const float* __restrict p3, const float* __restrict v3, PosPack* __restrict pos,
                                     float* __restrict vel, int64_t n, int np, int ng) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int qx, qy, qz;
     const int64_t i = ref_index(j, np, &qx, &qy, &qz);
-    PosSplit p;
-    p.cx = 0; p.cy = 0; p.cz = 0;
-    p.fx = p3[3 * i]; p.fy = p3[3 * i + 1]; p.fz = p3[3 * i + 2];
-    carry_wrap(p.cx, p.fx, ng);
-    carry_wrap(p.cy, p.fy, ng);
-    carry_wrap(p.cz, p.fz, ng);
-    pos[j] = pack(p);
+    PosPack q;
+    q.x = fx_from_double((double)p3[3 * i], ng);
+    q.y = fx_from_double((double)p3[3 * i + 1], ng);
+    q.z = fx_from_double((double)p3[3 * i + 2], ng);
+    pos[j] = q;
     vel[3 * j] = v3[3 * i]; vel[3 * j + 1] = v3[3 * i + 1]; vel[3 * j + 2] = v3[3 * i + 2];
 }
 
@@ -1154,31 +1129,21 @@ __global__ void k_to_reference(const PosPack* __restrict src, const float* __res
     vdst[3 * i] = vsrc[3 * j]; vdst[3 * i + 1] = vsrc[3 * j + 1]; vdst[3 * i + 2] = vsrc[3 * j + 2];
 }
 
-// double3 grid-unit positions in reference order (the FP64 path's layout) -> the split
-// layout in internal order: c = floor(x), f = float(x - c) (an f that rounds to 1
-// carries into the cell), velocities rounded to FP32.
+// double3 grid-unit positions in reference order (the FP64 path's layout) -> fixed point
+// in internal order, rint(x 2^22) mod ng 2^22; velocities rounded to FP32.
 __global__ void k_split_from_f64(const double* __restrict p3, const double* __restrict v3, PosPack* __restrict pos,
                                  float* __restrict vel, int64_t n, int np, int ng) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int qx, qy, qz;
     const int64_t i = ref_index(j, np, &qx, &qy, &qz);
-    int c[3];
-    float f[3];
+    PosPack q;
+    q.x = fx_from_double(p3[3 * i], ng);
+    q.y = fx_from_double(p3[3 * i + 1], ng);
+    q.z = fx_from_double(p3[3 * i + 2], ng);
+    pos[j] = q;
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double x = p3[3 * i + d];
-        const double fl = floor(x);
-        float r = (float)(x - fl);
-        int cc = (int)fl;
-        if (r >= 1.0f) { r = 0.0f; cc += 1; }
-        cc %= ng;
-        if (cc < 0) cc += ng;
-        c[d] = cc;
-        f[d] = r;
-        vel[3 * j + d] = (float)v3[3 * i + d];
-    }
-    pos[j] = pack(PosSplit{c[0], c[1], c[2], f[0], f[1], f[2]});
+    for (int d = 0; d < 3; d++) vel[3 * j + d] = (float)v3[3 * i + d];
 }
 // ... and back: x = (double)c + (double)f, velocities widened.
 __global__ void k_split_to_f64(const PosPack* __restrict pos, const float* __restrict vel, double* __restrict p3,
@@ -1259,22 +1224,16 @@ __global__ __launch_bounds__(CMP_NT) void k_compare(ParticleView a, ParticleView
 }
 
 // foldParticles (src/power_spectrum/cuda/fold_particles_power.cu:3-27): x -> 2 (x mod ng/2),
-// exact in the split layout.
-__device__ __forceinline__ void fold1(int& c, float& f, int half) {
-    c = c % half;
-    float f2 = f * 2.0f;
-    float fl = floorf(f2);
-    c = 2 * c + (int)fl;
-    f = f2 - fl;
-}
+// exact in fixed point: X -> 2 (X mod ng 2^21).
 __global__ void k_fold_split(PosPack* __restrict pos, int64_t n, int ng) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    PosSplit p = unpack(pos[i]);
-    fold1(p.cx, p.fx, ng / 2);
-    fold1(p.cy, p.fy, ng / 2);
-    fold1(p.cz, p.fz, ng / 2);
-    pos[i] = pack(p);
+    const unsigned half = (unsigned)ng << (HGPM_FRAC_BITS - 1);
+    PosPack q = pos[i];
+    q.x = 2u * (q.x % half);
+    q.y = 2u * (q.y % half);
+    q.z = 2u * (q.z % half);
+    pos[i] = q;
 }
 
 // Force of every cell (x, y, z order, 3 floats per cell) from the stored half potential of one box.

@@ -166,7 +166,10 @@ int hgpm_synchronize(hgpm_ctx* ctx);
 int hgpm_set_particles(hgpm_ctx* ctx, const float* pos, const float* vel);
 int hgpm_get_particles(hgpm_ctx* ctx, float* pos, float* vel);
 /* FP32 build layout in reference index order: cell 3*np^3 int32 in [0,ng),
-   offset 3*np^3 floats in [0,1), vel 3*np^3 floats. FP32 contexts only. */
+   offset 3*np^3 floats in [0,1), vel 3*np^3 floats. FP32 contexts only.  The engine
+   holds each coordinate in fixed point, cell * 2^22 + offset * 2^22: set rounds
+   cell + offset to the nearest 2^-22 cell (an offset within 2^-23 of 1 carries into the
+   next cell, wrapped), get returns the offsets exactly (multiples of 2^-22). */
 int hgpm_set_particles_split(hgpm_ctx* ctx, const int32_t* cell, const float* offset, const float* vel);
 int hgpm_get_particles_split(hgpm_ctx* ctx, int32_t* cell, float* offset, float* vel);
 /* FP64 layout (double3 AoS). FP64 contexts only. */

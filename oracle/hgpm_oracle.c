@@ -8,9 +8,10 @@
  * It restates the reference algorithm on the CPU in plain C11 (+ OpenMP), in three
  * modes that share the same control flow:
  *   ORC_F64   : every quantity in FP64 (the accuracy oracle, "the FP64 run").
- *   ORC_SPLIT : the build's FP32 layout and rounding recipe: int32 cell + FP32
- *               offset positions, FP32 CIC weights, int64 fixed-point density
- *               (2^-24 mass units), FP32 gather/kick/drift.  Density, cell indices
+ *   ORC_SPLIT : the build's FP32 layout and rounding recipe: positions in fixed
+ *               point (2^-22 cells), held here as int32 cell + FP32 offset on that
+ *               grid; FP32 CIC weights, int64 fixed-point density (2^-24 mass
+ *               units), FP32 gather/kick, fixed-point drift.  Density, cell indices
  *               and (given the same force grid) particle updates are the values
  *               the GPU must reproduce bit for bit.
  *   ORC_REF32 : the reference's own FP32 rounding (float3 positions, FP32 weights,
@@ -571,21 +572,24 @@ void orc_kick_ref32(int ng, int64_t n, const float* pos, float* vel, const float
 
 /* ------------------------------------------------------------------ drift ---- */
 /* UpdatePosKernel (src/pm_actions/cuda/update_positions_pm.cu:3-20) in the split
-   layout: f' = f + v*p (FP32), carry floor(f') into the int32 cell, wrap mod ng. */
+   layout, whose positions are fixed point X = cell 2^22 + offset 2^22 (offsets on the
+   2^-22 grid): the FP32 displacement t = v*p, split into ti = trunc(t) and the fraction
+   t - ti (both exact), is added as ti 2^22 + rint((t - ti) 2^22) (ties to even) and X
+   wrapped modulo ng 2^22.  A move of a box or more is first reduced modulo ng in FP32. */
 void orc_drift_split(int ng, int64_t n, int32_t* cell, float* off, const float* vel, float pf) {
+    const int64_t span = (int64_t)ng << 22;
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; p++) {
         for (int d = 0; d < 3; d++) {
             float t = vel[3 * p + d] * pf;
-            float f = off[3 * p + d] + t;
-            float fl = floorf(f);
-            float r = f - fl;
-            int c = cell[3 * p + d] + (int)fl;
-            if (r >= 1.0f) { r = 0.0f; c += 1; }
-            c %= ng;
-            if (c < 0) c += ng;
-            cell[3 * p + d] = c;
-            off[3 * p + d] = r;
+            if (!(fabsf(t) < (float)ng)) t = t - (float)ng * truncf(t / (float)ng);
+            const float ti = truncf(t);
+            int64_t x = ((int64_t)cell[3 * p + d] << 22) + (int64_t)(off[3 * p + d] * 4194304.0f);
+            x += ((int64_t)(int)ti << 22) + (int64_t)(int)rintf((t - ti) * 4194304.0f);
+            if (x < 0) x += span;
+            else if (x >= span) x -= span;
+            cell[3 * p + d] = (int32_t)(x >> 22);
+            off[3 * p + d] = (float)(x & 4194303) * (1.0f / 4194304.0f);
         }
     }
 }
@@ -788,19 +792,15 @@ void orc_ic_f64(int ng, int np, double rl, uint32_t seed, const double* ipk, int
     free(work);
 }
 
-/* Split a double position into the build layout (cell, offset) exactly as the GPU
-   IC does: c = floor(x), f = float(x - c), f == 1 carries. */
+/* A double position in the build layout exactly as the GPU places it: the fixed-point
+   coordinate rint(x 2^22) mod ng 2^22, returned as (cell, offset on the 2^-22 grid). */
 void orc_split_positions(int ng, int64_t n, const double* pos, int32_t* cell, float* off) {
+    const int64_t span = (int64_t)ng << 22;
     for (int64_t p = 0; p < 3 * n; p++) {
-        double x = pos[p];
-        double fl = floor(x);
-        float f = (float)(x - fl);
-        int c = (int)fl;
-        if (f >= 1.0f) { f = 0.0f; c += 1; }
-        c %= ng;
-        if (c < 0) c += ng;
-        cell[p] = c;
-        off[p] = f;
+        int64_t x = (int64_t)rint(pos[p] * 4194304.0) % span;
+        if (x < 0) x += span;
+        cell[p] = (int32_t)(x >> 22);
+        off[p] = (float)(x & 4194303) * (1.0f / 4194304.0f);
     }
 }
 

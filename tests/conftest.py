@@ -29,19 +29,23 @@ def orc():
     return oracle
 
 
+def split_of(x, ng):
+    """Grid-unit positions (FP64) in the split layout as the engine holds them: the
+    fixed-point coordinate rint(x 2^22) mod ng 2^22 as (cell int32, offset float32 on the
+    2^-22 grid)."""
+    X = np.mod(np.rint(np.asarray(x, np.float64) * 2.0 ** 22).astype(np.int64), np.int64(ng) << 22)
+    return (X >> 22).astype(np.int32), ((X & ((1 << 22) - 1)).astype(np.float64) * 2.0 ** -22).astype(np.float32)
+
+
 def rand_split(rng, n, ng, clustered=False):
-    """Random particles in the split layout (cell int32, offset float32 in [0,1))."""
+    """Random particles in the split layout (cell int32, offset float32 in [0,1) on the
+    2^-22 grid)."""
     if clustered:
         centre = rng.integers(0, ng, 3)
         x = (centre + rng.normal(0, 1.5, (n, 3))) % ng
     else:
         x = rng.random((n, 3)) * ng
-    cell = np.floor(x).astype(np.int32) % ng
-    off = (x - np.floor(x)).astype(np.float32)
-    bad = off >= 1.0
-    off[bad] = 0.0
-    cell[bad] = (cell[bad] + 1) % ng
-    return cell, off
+    return split_of(x, ng)
 
 
 def lattice_split(rng, np_, ng, sigma, outliers=0):
@@ -52,9 +56,4 @@ def lattice_split(rng, np_, ng, sigma, outliers=0):
     x = (q + rng.normal(0, sigma, q.shape)) % ng
     if outliers:
         x[:outliers] = rng.random((outliers, 3)) * ng
-    cell = np.floor(x).astype(np.int32) % ng
-    off = (x - np.floor(x)).astype(np.float32)
-    bad = off >= 1.0
-    off[bad] = 0.0
-    cell[bad] = (cell[bad] + 1) % ng
-    return cell, off
+    return split_of(x, ng)

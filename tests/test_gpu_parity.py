@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import lattice_split, rand_split
+from conftest import lattice_split, rand_split, split_of
 
 pytestmark = pytest.mark.gpu
 
@@ -41,9 +41,10 @@ def test_deposit_bit_exact(hg, orc, ng, clustered):
     rng = np.random.default_rng(ng + clustered)
     with mk(hg, ng) as m:
         cell, off = rand_split(rng, m.N, ng, clustered)
-        # particles on the periodic seam and exactly on lattice points
+        # particles on the periodic seam (one position step below the box edge) and
+        # exactly on lattice points
         cell[:5] = ng - 1
-        off[:5] = np.float32(1.0) - np.float32(2.0 ** -24)
+        off[:5] = np.float32(1.0) - np.float32(2.0 ** -22)
         off[5:9] = 0.0
         vel = np.zeros((m.N, 3), np.float32)
         m.set_particles_split(cell, off, vel)
@@ -104,9 +105,7 @@ def test_deposit_dense_cells_take_the_wide_route(hg, orc, np_):
     p = hg.make_params(ng, np_, 2.0 * ng, 50.0, 0.0, 100)
     with hg.MemoryManager(p) as m, hg.MemoryManager(p) as m2:
         x = (np.array([20.0, 33.0, 41.0]) + rng.normal(0, 0.3, (m.N, 3))) % ng
-        cell = np.floor(x).astype(np.int32) % ng
-        off = (x - np.floor(x)).astype(np.float32)
-        off[off >= 1.0] = 0.0
+        cell, off = split_of(x, ng)
         vel = np.zeros((m.N, 3), np.float32)  # at rest: the first step deposits the clump
         ref = orc.cic_split(ng, cell, off, (ng / np_) ** 3)
         m.set_particles_split(cell, off, vel)
@@ -223,9 +222,7 @@ def test_deposit_heavy_particles(hg, orc, ng, np_, sigma):
         cell, off = lattice_split(rng, np_, ng, sigma, outliers=2)
         if sigma < 1:  # a clump: 4096 particles piled into a few cells
             x = (np.array([50.0, 70.0, 90.0]) + rng.normal(0, 0.4, (4096, 3))) % ng
-            cell[:4096] = np.floor(x).astype(np.int32) % ng
-            off[:4096] = (x - np.floor(x)).astype(np.float32)
-            off[off >= 1.0] = 0.0
+            cell[:4096], off[:4096] = split_of(x, ng)
         vel = rng.normal(0, 0.5, (m.N, 3)).astype(np.float32)
         ref = orc.cic_split(ng, cell, off, mass)
         m.set_particles_split(cell, off, vel)
