@@ -547,7 +547,7 @@ def main():
         "data": "synthetic: Zel'dovich ICs at z=50 from Philox white noise and an Eisenstein-Hu no-wiggle P(k)",
         "config": {"workload": workload, "np": ng, "ng": ng,
                    "rl": args.rl, "z_ini": 50, "z_fin": 0, "steps_in_run": n_run,
-                   "timed_steps": [args.warmup, args.warmup + args.steps - 1], "positions": "int32 cell + f32 offset",
+                   "timed_steps": [args.warmup, args.warmup + args.steps - 1], "positions": "fixed point, 3 x uint32 in 2^-22 cells",
                    "deposit": "int64 fixed point 2^-24"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
