@@ -259,3 +259,50 @@ def test_kick_split_against_f64(orc):
     assert err.max() <= 4e-7 * (np.abs(v0).max() + np.abs(dv).max())
     # the kick is not the identity: a dropped factor or a lost corner fails here
     assert np.abs(dv).max() > 1e-3
+
+
+def test_drift_split_fixed_point(orc):
+    """The split drift moves a fixed-point coordinate (2^-22 cells) by the FP32
+    displacement v * p rounded to the nearest 2^-22 cell.  Pins: a displacement that is
+    a multiple of 2^-22 moves exactly by it; ties go to even (a half step does not move,
+    one and a half steps move two); a drift by -t undoes a drift by +t exactly; moves
+    wrap modulo ng; and any drift agrees with the FP64 drift of the same displacement
+    to within half a step."""
+    ng = 16
+    u = 2.0 ** -22
+    cell = np.array([[3, 15, 0]], np.int32)
+    off = np.array([[0.25, 1.0 - u, 0.0]], np.float32)
+    one = np.ones((1, 3), np.float32)
+
+    def drift(c, o, t):
+        return orc.drift_split(ng, c, o, one * np.float32(t), np.float32(1.0))
+
+    # exact multiples of the step, across the periodic edge in y and below 0 in z
+    c, o = drift(cell, off, 5 * u)
+    x = c + o.astype(np.float64)
+    assert np.array_equal(x, np.array([[3.25 + 5 * u, (5 - 1) * u, 5 * u]]))
+    c, o = drift(cell, off, -3 * u)
+    x = c + o.astype(np.float64)
+    assert np.array_equal(x, np.array([[3.25 - 3 * u, 16 - 4 * u, 16 - 3 * u]]))
+    # ties to even: 0.5 step -> 0, 1.5 steps -> 2, 2.5 steps -> 2, -0.5 step -> 0
+    c3, o3 = np.full((1, 3), 3, np.int32), np.full((1, 3), 0.25, np.float32)
+    for t, k in ((0.5, 0), (1.5, 2), (2.5, 2), (-0.5, 0), (-1.5, -2)):
+        c, o = drift(c3, o3, t * u)
+        assert np.all(c + o.astype(np.float64) == 3.25 + k * u), (t, c, o)
+    # whole cells plus a fraction, and more than a box (reduced modulo ng)
+    c, o = drift(cell, off, 7.75)
+    assert np.array_equal(c + o.astype(np.float64), np.array([[11.0, 7.75 - u, 7.75]]))
+    c, o = drift(cell, off, 16.0 + 2.0)
+    assert np.array_equal(c, np.array([[5, 1, 2]])) and np.array_equal(o, np.array([[0.25, 1 - u, 0.0]], np.float32))
+    # reversible, and within half a step of the FP64 drift
+    rng = np.random.default_rng(41)
+    cs, os_ = rand_split(rng, 5000, ng)
+    vel = rng.normal(0, 3, (5000, 3)).astype(np.float32)
+    for pf in (0.37, 1.9, 0.013):
+        c1, o1 = orc.drift_split(ng, cs, os_, vel, np.float32(pf))
+        c2, o2 = orc.drift_split(ng, c1, o1, -vel, np.float32(pf))
+        assert np.array_equal(c2, cs) and np.array_equal(o2, os_)
+        t = (vel * np.float32(pf)).astype(np.float64)
+        want = (cs + os_.astype(np.float64) + t) % ng
+        d = (c1 + o1.astype(np.float64) - want + ng / 2) % ng - ng / 2
+        assert np.abs(d).max() <= u / 2 * (1 + 1e-9)
