@@ -170,7 +170,9 @@ __global__ __launch_bounds__(NTC) void k_x_psi(float2* __restrict h1, const floa
     fft_tile<float2, N, CC, NTC, +1, true, false>(
         [&](int b, int r, int n, int) {
             const float ssum = hs[n] + hyz;
-            const float g = (n | y | z) == 0 ? 0.0f : gcoef / ssum;
+            // v_rcp_f32 (1 ulp) for the quotient: ssum >= sin^2(pi / N) is far from the
+            // denormal range, and the solve is checked to a tolerance, not bit for bit
+            const float g = (n | y | z) == 0 ? 0.0f : gcoef * __builtin_amdgcn_rcpf(ssum);
             return make_float2(g * R[b][r].x, g * R[b][r].y);
         },
         [&](int, int, int n, int c, float2 v) { h1[base + n * NP + c] = v; }, lds, twr);
@@ -186,23 +188,41 @@ __device__ __forceinline__ float2 herm(const float2* __restrict row, int n) {
 
 // Inverse z transform of real rows from their half spectra, rows in pairs: the
 // Hermitian extensions of a and b as one complex transform of a + i b, whose real and
-// imaginary parts are the rows.
+// imaginary parts are the rows.  When the last stage's butterflies per row are fewer
+// than a wave (N / R_last < 64: lengths with a radix-3 or radix-16 first stage, and
+// N <= 256) its direct stores would scatter each wave over several rows in short
+// pieces; the rows then go through LDS and are stored whole, lane after lane.
 template <int RC>
 __global__ __launch_bounds__(NT) void k_rows_c2r_real(const float2* __restrict hz, float* __restrict out,
                                                       const float2* __restrict tw) {
     constexpr int PC = RC / 2;
+    constexpr bool STAGED = N / Plan<N>::R(Plan<N>::S - 1) < 64;
     __shared__ float2 lds[lds_elems<N, PC>()];
     const size_t row0 = (size_t)blockIdx.x * RC;
-    fft_tile<float2, N, PC, NT, +1, false, true>(
-        [&](int, int, int n, int c) {
-            const float2 a = herm(hz + hrow(row0 + 2 * c), n), b = herm(hz + hrow(row0 + 2 * c + 1), n);
-            return make_float2(a.x - b.y, a.y + b.x);
-        },
-        [&](int, int, int n, int c, float2 v) {
+    auto ld = [&](int, int, int n, int c) {
+        const float2 a = herm(hz + hrow(row0 + 2 * c), n), b = herm(hz + hrow(row0 + 2 * c + 1), n);
+        return make_float2(a.x - b.y, a.y + b.x);
+    };
+    if constexpr (!STAGED) {
+        fft_tile<float2, N, PC, NT, +1, false, true>(
+            ld,
+            [&](int, int, int n, int c, float2 v) {
+                out[(row0 + 2 * c) * N + n] = v.x;
+                out[(row0 + 2 * c + 1) * N + n] = v.y;
+            },
+            lds, tw);
+    } else {
+        // the last stage has read all its inputs before the first output
+        fft_tile<float2, N, PC, NT, +1, false, true>(
+            ld, [&](int, int, int n, int c, float2 v) { lds[lds_slot<N, PC, true>(n, c)] = v; }, lds, tw);
+        __syncthreads();
+        for (int i = threadIdx.x; i < PC * N; i += NT) {
+            const int c = i / N, n = i - c * N;
+            const float2 v = lds[lds_slot<N, PC, true>(n, c)];
             out[(row0 + 2 * c) * N + n] = v.x;
             out[(row0 + 2 * c + 1) * N + n] = v.y;
-        },
-        lds, tw);
+        }
+    }
 }
 
 // --------------------------------------------------------- launchers ----
