@@ -595,8 +595,8 @@ int hgpm_set_particles(hgpm_ctx* c, const float* pos, const float* vel) {
     const size_t n3 = 3 * (size_t)c->N;
     c->boxes_valid = false;
     if (c->prec == HGPM_FP32) {
-        // the device split (carry_wrap) converts floor(x) to int: reject what it cannot
-        // hold, and NaN / Inf, before anything reaches the device
+        // the device conversion to fixed point takes rint(x 2^22) as a 64-bit integer:
+        // reject what it cannot hold, and NaN / Inf, before anything reaches the device
         for (size_t i = 0; i < n3; i++)
             if (!(std::fabs(pos[i]) < 1073741824.0f) || !std::isfinite(vel[i]))
                 return fail(c, HGPM_ERR_ARG, "non-finite particle (or |x| >= 2^30) at index " + std::to_string(i / 3));

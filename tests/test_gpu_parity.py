@@ -839,6 +839,26 @@ def test_ensemble_matches_single_boxes(hg):
 
 
 
+def test_ensemble_refuses_boxes_without_particles(hg):
+    """A batch shares its buffers, but readiness is per box: with one box of a batch
+    generated and the other not, stepping, running and reading the unset box are
+    refused with HGPM_ERR_STATE (nothing is launched on its uninitialised memory)."""
+    case("ensemble 2 x 32^3, one box without particles")
+    ps = [hg.make_params(32, 32, 64.0, 30.0, 0.0, 40, seed=21 + b) for b in range(2)]
+    with hg.Ensemble(ps) as e:
+        assert e.batches()[0] == 1
+        hdr, vals, delta, dot_delta, ts = _ic_inputs(hg, ps[0])
+        e.generate_ic(0, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+        co = [hg.schedule(p, 2, merge=True)[0] for p in ps]
+        for call in (lambda: e.step(co), lambda: e.run([co]), lambda: e.get_particles(1)):
+            with pytest.raises(hg.HgpmError) as err:
+                call()
+            assert err.value.code == hg.ERR_STATE
+        e.get_particles(0)  # the generated box reads back
+        e.generate_ic(1, hdr, vals, delta, dot_delta, ts.fscal, ts.deltaT)
+        e.step(co)
+        e.synchronize()
+
 def test_ensemble_graph_run_matches_steps(hg):
     """Steps captured into one HIP graph per run (hgpm_ensemble_run) against the same
     steps launched one by one: bit-identical particles and P(k) after each of three
