@@ -60,6 +60,7 @@ struct hgpm_ctx {
     unsigned long long arena_alloc = 0, arena_cap = 0; // cells allocated / usable
     void* dslots = nullptr;
     unsigned* dcount = nullptr;
+    unsigned* dredo = nullptr; // owner blocks listed for the 64-bit owner pass (after dcount, one allocation)
     unsigned* deferred = nullptr;
     unsigned* wide = nullptr;
     unsigned short* wide_slots = nullptr;
@@ -234,6 +235,7 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.cap = c->arena_cap;
     b.slots = c->dslots;
     b.count = c->dcount;
+    b.redo = c->dredo;
     b.nslots = c->nslots;
     b.deferred = c->deferred;
     b.wide = c->wide;
@@ -488,7 +490,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
             c->arena_cap = c->arena_alloc;
             if ((rc = dalloc(c, &c->arena, sizeof(unsigned) * c->arena_alloc, "deposit arena")) ||
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
-                (rc = dalloc(c, &c->dcount, sizeof(unsigned) * nblk, "deposit counts")) ||
+                (rc = dalloc(c, &c->dcount, sizeof(unsigned) * 2 * nblk, "deposit counts and redo list")) ||
                 (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||
                 (rc = dalloc(c, &c->wide, sizeof(unsigned) * ntiles, "deposit 64-bit list")) ||
                 (rc = dalloc(c, &c->wide_slots, sizeof(unsigned short) * HGPM_DEP_WKEEP * ntiles,
@@ -496,6 +498,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
                 (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
                 (rc = dalloc(c, (char**)&c->boxes, 32 * ntiles, "gather boxes")))
                 return bail(rc);
+            c->dredo = c->dcount + nblk;
             // particles start at cell 0, offset 0 (valid if never set; a step refuses them)
             if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess ||
                 hipMemset(c->pos, 0, sizeof(PosPackH) * NB) != hipSuccess ||
@@ -1375,16 +1378,17 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
         *value = (int)(t >> 20);
         return HGPM_OK;
     }
-    // last deposit: tiles that took the atomic route, tiles whose windows were stored
-    // with 64-bit cells, deferred tiles that found their slot list or sub-arena full
-    const char* dkeys[3] = {"deposit_deferred", "deposit_wide", "deposit_deferred_full"};
-    for (int i = 0; i < 3; i++)
+    // last deposit: owner blocks summed by the 64-bit owner pass, tiles that took the
+    // atomic route, tiles whose windows were stored with 64-bit cells, deferred tiles
+    // that found their slot list or sub-arena full
+    const char* dkeys[4] = {"deposit_owner_redo", "deposit_deferred", "deposit_wide", "deposit_deferred_full"};
+    for (int i = 0; i < 4; i++)
         if (!strcmp(key, dkeys[i])) {
             if (c->prec != HGPM_FP32) { *value = 0; return HGPM_OK; }
             unsigned long long v[4];
             HIPCHK(c, hipMemcpyAsync(v, c->ctl, sizeof(v), hipMemcpyDeviceToHost, c->st));
             HIPCHK(c, hipStreamSynchronize(c->st));
-            *value = (int)v[1 + i];
+            *value = (int)v[i];
             return HGPM_OK;
         }
     return fail(c, HGPM_ERR_ARG, std::string("unknown tunable ") + key);
@@ -1593,7 +1597,7 @@ size_t batch_bytes(const hgpm_params& p, int nstore, int nwave) {
     size_t b = (sizeof(PosPackH) + 3 * sizeof(float)) * N * nstore;     // particles
     b += (2 * sizeof(float2) + sizeof(float)) * G * nwave;              // A, B, psi
     b += 6 * sizeof(unsigned) * G * nwave;                              // arena
-    b += (16 * HGPM_DEP_SLOTS + sizeof(unsigned)) * nblk;               // slots, counts
+    b += (16 * HGPM_DEP_SLOTS + 2 * sizeof(unsigned)) * nblk;           // slots, counts, redo list
     b += (2 * sizeof(unsigned) + sizeof(unsigned long long) + 32 + 2 * HGPM_DEP_WKEEP) * ntiles; // tile buffers
     b += 64ull << 20;                                                   // tables, P(k), slack
     return b;

@@ -63,8 +63,11 @@ void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf
 //              mass units stores its window again with 64-bit cells);
 //   slots    : HGPM_DEP_SLOTS chunk descriptors (16 B) per 16^3 owner block;
 //   count    : chunks listed per owner block, zero between deposits;
+//   redo     : one entry per owner block: the blocks the 32-bit owner pass hands to the
+//              64-bit one (a 64-bit chunk, or a cell sum of 2^32 units or more);
 //   deferred : one entry per tile;
-//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [1] deferred tiles, of
+//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] entries of redo,
+//              [1] deferred tiles, of
 //              them [3] for a full slot list or sub-arena (the others had too many
 //              chunks); [2] tiles stored with 64-bit cells; [4 + s] cells reserved in
 //              sub-arena s;
@@ -80,6 +83,7 @@ struct DepositBuffers {
     unsigned long long cap;
     void* slots;
     unsigned* count;
+    unsigned* redo;
     int nslots;
     unsigned* deferred;
     unsigned* wide; // tiles stored again with 64-bit cells: one entry per tile

@@ -89,7 +89,6 @@ constexpr int DEP_MAXCH = 512;             // chunks per tile on the arena route
 constexpr int DEP_WKEEP = HGPM_DEP_WKEEP;  // slots kept per tile for the 64-bit pass
 static_assert(DEP_MAXCH <= DEP_NT, "one chunk per thread");
 constexpr int OWN_NT = 256;
-constexpr int OWN_U = 16;               // arena loads in flight per lane
 constexpr int DEP_NSUB = HGPM_DEP_NSUB; // sub-arenas
 constexpr int DEP_CTL_SUB = 4;          // first sub-arena counter in ctl
 static_assert(DEP_TILE == HGPM_TL * HGPM_TL * HGPM_TL, "tile");
@@ -626,47 +625,64 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_wide(DepWindowArgs g) {
     }
 }
 
-// One workgroup per OB^3 owner block: sum the listed chunks in LDS, write every cell.
-// Resets the block's chunk count for the next deposit.  A wave takes one chunk at a
+// Owner pass: one workgroup per OB^3 owner block sums the chunks listed for it in LDS
+// and writes every cell of the block once (the grid is not cleared beforehand); the
+// block's chunk count is reset for the next deposit.  A wave takes one chunk at a
 // time: each lane holds up to OWN_J cells of one (y, z) plane of the chunk (p = lane +
-// 64 j, decoded once per chunk) and walks the chunk's x planes, so a cell costs an
-// add for its arena address and one for its LDS address; OWN_U planes are loaded
-// before the first LDS add so that their loads are in flight together.
+// 64 j, decoded once per chunk) and walks the chunk's x planes, so a cell costs an add
+// for its arena address and one for its LDS address; U planes are loaded before the
+// first LDS add so that their loads are in flight together.
+// Two kernels: k_deposit_owner sums in uint32 LDS cells (16 KB a block, few registers:
+// eight waves per SIMD) and hands a block to k_deposit_owner64 (int64 LDS cells) when
+// one of its chunks has 64-bit cells or a cell's sum reached 2^32 (its stored cells
+// then sum to less than the values added).  A handed-over block writes nothing and
+// keeps its count; k_deposit_owner64 takes the blocks listed in redo (ctl[0] of them).
 // rhof (may be null): when no tile was deferred (ctl[1] == 0) the block is written as
 // FP32 (q - mean) * 2^-24 to rhof, the form the forward z transform takes, instead of
 // int64 q to rho; mean = totals[box] / ng^3 (the totals of every tile are final).
-__global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __restrict arena,
-                                                          const uint4* __restrict slots,
-                                                          unsigned* __restrict count, int nslots, int ng,
-                                                          unsigned long long* __restrict rho, float* __restrict rhof,
-                                                          const unsigned long long* __restrict ctl,
-                                                          const unsigned long long* __restrict totals, int nblk,
-                                                          int64_t gcells) {
+struct OwnerArgs {
+    const unsigned* __restrict arena;
+    const uint4* __restrict slots;
+    unsigned* __restrict count;
+    unsigned* __restrict redo;
+    int nslots, ng;
+    unsigned long long* __restrict rho;
+    float* __restrict rhof;
+    const unsigned long long* __restrict ctl;
+    unsigned long long* __restrict nredo; // = ctl[0]
+    const unsigned long long* __restrict totals;
+    int nblk;
+    int64_t gcells;
+};
+constexpr int OWN_U32 = 8;  // arena loads in flight per lane, 32-bit pass
+constexpr int OWN_U64 = 16; // 64-bit pass
+// Returns false when the block needs the 64-bit pass (ACC = unsigned only).
+template <typename ACC, int U>
+__device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* acc, uint4* desc, int* s_n, int* s_redo,
+                                            unsigned long long* s_dif) {
+    constexpr bool A64 = sizeof(ACC) == 8;
     constexpr int OWN_J = OB * OB / 64; // plane slots per lane (a plane holds <= OB^2 cells)
-    __shared__ unsigned long long acc[OB * OB * OB];
-    __shared__ uint4 desc[HGPM_DEP_SLOTS];
-    __shared__ int s_n;
-    const int nb = ng / OB;
-    const int blk = (int)xcd_remap(blockIdx.x, gridDim.x); // over every box of the batch
-    const int box = blk / nblk, lb = blk - box * nblk;
-    rho += (int64_t)box * gcells;
-    const bool f32 = rhof != nullptr && ctl[1] == 0ull;
-    const double mean = f32 ? (double)(long long)totals[box] / (double)gcells : 0.0;
-    if (f32) rhof += (int64_t)box * gcells;
+    const int nb = g.ng / OB;
+    const int box = blk / g.nblk, lb = blk - box * g.nblk;
+    const bool f32 = g.rhof != nullptr && g.ctl[1] == 0ull;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // every slot descriptor is loaded together with the count (the unused ones are
     // ignored), so the two loads cost one memory latency
     uint4 dl = make_uint4(0u, 0u, 0x80000000u, 0u);
-    if (tid < nslots) dl = slots[(size_t)blk * nslots + tid];
+    if (tid < g.nslots) dl = g.slots[(size_t)blk * g.nslots + tid];
     if (tid == 0) {
-        const unsigned c = count[blk];
-        count[blk] = 0u;
-        s_n = (int)min(c, (unsigned)nslots);
+        const unsigned c = g.count[blk];
+        if (A64) g.count[blk] = 0u;
+        *s_n = (int)min(c, (unsigned)g.nslots);
+        *s_redo = 0;
+        *s_dif = 0ull;
     }
-    if (tid < nslots) desc[tid] = dl;
-    for (int i = tid; i < OB * OB * OB; i += OWN_NT) acc[i] = 0ull;
+    if (tid < g.nslots) desc[tid] = dl;
+    for (int i = tid; i < OB * OB * OB; i += OWN_NT) acc[i] = (ACC)0;
     __syncthreads();
-    const int nd = s_n;
+    const int nd = *s_n;
+    bool wrap = false;             // a chunk with 64-bit cells (32-bit pass)
+    unsigned long long added = 0;  // this lane's sum of the values added (32-bit pass)
     for (int d = wv; d < nd; d += OWN_NT / 64) {
         // one descriptor per wave: its words are uniform (scalar registers)
         uint4 de = desc[d];
@@ -675,7 +691,11 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
         de.z = __builtin_amdgcn_readfirstlane(de.z);
         de.w = __builtin_amdgcn_readfirstlane(de.w);
         if (de.z >> 31) continue;
-        const unsigned* __restrict ch = arena + ((unsigned long long)de.x | ((unsigned long long)de.y << 32));
+        if (!A64 && (de.z & 0x40000000u)) { // 64-bit cells: the 64-bit pass
+            wrap = true;
+            break;
+        }
+        const unsigned* __restrict ch = g.arena + ((unsigned long long)de.x | ((unsigned long long)de.y << 32));
         const int ox = de.z & 15, oy = (de.z >> 4) & 15, oz = (de.z >> 8) & 15;
         const int mx = ((de.z >> 12) & 15) + 1, my = ((de.z >> 16) & 15) + 1, mz = ((de.z >> 20) & 15) + 1;
         const int e2 = (int)(de.w >> 16);
@@ -684,62 +704,111 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(const unsigned* __rest
         // p -> (ly, lz) by multiplication with a rounded reciprocal: exact here because
         // p < 256 keeps the rounding error below the 0.5/mz margin of (p + 0.5)
         const float rmz = 1.0f / (float)mz;
-        for (int j = 0; j * 64 < myz; j++) {
+        for (int j = 0; j < OWN_J && j * 64 < myz; j++) {
             const int pl = lane + 64 * j;
             const bool pv = pl < myz;
             const int ly = (int)(((float)pl + 0.5f) * rmz), lz = pl - ly * mz;
             const unsigned po = pv ? (unsigned)(ly * e2 + lz) : 0u; // arena offset in the plane
             const int la = ox * OB * OB + (oy + ly) * OB + oz + lz; // LDS cell of x plane 0
-            if (de.z & 0x40000000u) {
+            if (A64 && (de.z & 0x40000000u)) {
                 // a window of 64-bit cells (two arena words per cell)
-                for (int x0 = 0; x0 < mx; x0 += OWN_U) {
-                    unsigned long long v[OWN_U];
+                for (int x0 = 0; x0 < mx; x0 += U) {
+                    unsigned long long v[U];
 #pragma unroll
-                    for (int u = 0; u < OWN_U; u++)
+                    for (int u = 0; u < U; u++)
                         if (x0 + u < mx)
-                            v[u] = *(const unsigned long long*)((const char*)ch +
-                                                               ((unsigned)(x0 + u) * e12 + po) * 8u);
+                            v[u] = *(const unsigned long long*)((const char*)ch + ((unsigned)(x0 + u) * e12 + po) * 8u);
 #pragma unroll
-                    for (int u = 0; u < OWN_U; u++)
-                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], v[u]);
+                    for (int u = 0; u < U; u++)
+                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (ACC)v[u]);
                 }
                 continue;
             }
-            for (int x0 = 0; x0 < mx; x0 += OWN_U) {
-                unsigned v[OWN_U];
+            for (int x0 = 0; x0 < mx; x0 += U) {
+                unsigned v[U];
 #pragma unroll
-                for (int u = 0; u < OWN_U; u++) {
-                    // uniform bound; lanes past the plane load its first cell.  A chunk
-                    // lies in a window of at most DEP_MAXCH owner blocks: byte offsets
-                    // from its first cell fit 32 bits (also for 64-bit cells)
-                    if (x0 + u < mx)
-                        v[u] = *(const unsigned*)((const char*)ch + ((unsigned)(x0 + u) * e12 + po) * 4u);
+                for (int u = 0; u < U; u++) {
+                    // every slot loads (planes past the chunk's last load that one again,
+                    // lanes past the plane its first cell) so that the U loads issue
+                    // together.  A chunk lies in a window of at most DEP_MAXCH owner
+                    // blocks: byte offsets from its first cell fit 32 bits (also for
+                    // 64-bit cells)
+                    const unsigned xu = (unsigned)min(x0 + u, mx - 1);
+                    v[u] = *(const unsigned*)((const char*)ch + (xu * e12 + po) * 4u);
                 }
+                if constexpr (A64) {
 #pragma unroll
-                for (int u = 0; u < OWN_U; u++) {
-                    if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (unsigned long long)v[u]);
+                    for (int u = 0; u < U; u++)
+                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (unsigned long long)v[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (x0 + u < mx && pv && v[u]) {
+                            atomicAdd(&acc[la + (x0 + u) * OB * OB], v[u]);
+                            added += v[u];
+                        }
                 }
             }
         }
     }
+    if (!A64 && wrap) *s_redo = 1;
     __syncthreads();
     // thread tid writes (x, y, z) = (k, t / OB, t % OB), t = tid % OB^2, of the block for
     // k = tid / OB^2, tid / OB^2 + KS, ...
     constexpr int KS = OWN_NT / (OB * OB);
     static_assert(OWN_NT % (OB * OB) == 0 && OB % KS == 0, "whole (y, z) planes per pass");
     const int t = tid % (OB * OB), k0 = tid / (OB * OB);
+    if constexpr (!A64) {
+        // a cell whose sum reached 2^32 wrapped: the stored cells then sum to less than
+        // the values added (an exact test, one add per cell); the block goes to the
+        // 64-bit pass
+        unsigned long long stored = 0;
+        for (int k = k0; k < OB; k += KS) stored += acc[k * OB * OB + t];
+        // the block's sum of (added - stored), modulo 2^64: k wraps leave k 2^32 < 2^64
+        unsigned long long dif = added - stored;
+        for (int off = 32; off > 0; off >>= 1) dif += __shfl_xor(dif, off, 64);
+        if (lane == 0) atomicAdd(s_dif, dif);
+        __syncthreads();
+        if (*s_redo || *s_dif) {
+            if (tid == 0) g.redo[atomicAdd(g.nredo, 1ull)] = (unsigned)blk;
+            return false;
+        }
+    }
     const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
-    const int64_t at = gidx(ng, bx * OB, by * OB + (t >> 4), bz * OB + (t & 15));
-    const int64_t plane = (int64_t)ng * ng;
+    const int64_t at = gidx(g.ng, bx * OB, by * OB + (t >> 4), bz * OB + (t & 15));
+    const int64_t plane = (int64_t)g.ng * g.ng;
     if (f32) {
+        const double mean = (double)(long long)g.totals[box] / (double)g.gcells;
+        float* __restrict rf = g.rhof + (int64_t)box * g.gcells;
 #pragma unroll 4
         for (int k = k0; k < OB; k += KS) {
-            const double q = (double)(long long)acc[k * OB * OB + t];
-            rhof[at + k * plane] = (float)((q - mean) * (1.0 / 16777216.0));
+            const double q = (double)(long long)(unsigned long long)acc[k * OB * OB + t];
+            rf[at + k * plane] = (float)((q - mean) * (1.0 / 16777216.0));
         }
     } else {
+        unsigned long long* __restrict r = g.rho + (int64_t)box * g.gcells;
 #pragma unroll 4
-        for (int k = k0; k < OB; k += KS) rho[at + k * plane] = acc[k * OB * OB + t];
+        for (int k = k0; k < OB; k += KS) r[at + k * plane] = (unsigned long long)acc[k * OB * OB + t];
+    }
+    if (!A64 && tid == 0) g.count[blk] = 0u;
+    return true;
+}
+__global__ __launch_bounds__(OWN_NT) void k_deposit_owner(OwnerArgs g) {
+    __shared__ unsigned acc[OB * OB * OB];
+    __shared__ uint4 desc[HGPM_DEP_SLOTS];
+    __shared__ int s_n, s_redo;
+    __shared__ unsigned long long s_dif;
+    owner_block<unsigned, OWN_U32>(g, (int)xcd_remap(blockIdx.x, gridDim.x), acc, desc, &s_n, &s_redo, &s_dif);
+}
+__global__ __launch_bounds__(OWN_NT) void k_deposit_owner64(OwnerArgs g) {
+    __shared__ unsigned long long acc[OB * OB * OB];
+    __shared__ uint4 desc[HGPM_DEP_SLOTS];
+    __shared__ int s_n, s_redo;
+    __shared__ unsigned long long s_dif;
+    const unsigned long long n = *g.nredo;
+    for (unsigned long long q = blockIdx.x; q < n; q += gridDim.x) {
+        owner_block<unsigned long long, OWN_U64>(g, (int)g.redo[q], acc, desc, &s_n, &s_redo, &s_dif);
+        __syncthreads();
     }
 }
 
@@ -1320,9 +1389,25 @@ void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, 
 void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
     const int nblk = (int)(nb * nb * nb);
-    k_deposit_owner<<<nb * nb * nb * b.nbox, OWN_NT, 0, s>>>(b.arena, (const uint4*)b.slots, b.count, b.nslots, ng,
-                                                             (unsigned long long*)rho, rhof, b.ctl, b.totals, nblk,
-                                                             (int64_t)ng * ng * ng);
+    OwnerArgs g;
+    g.arena = b.arena;
+    g.slots = (const uint4*)b.slots;
+    g.count = b.count;
+    g.redo = b.redo;
+    g.nslots = b.nslots;
+    g.ng = ng;
+    g.rho = (unsigned long long*)rho;
+    g.rhof = rhof;
+    g.ctl = b.ctl;
+    g.nredo = b.ctl; // ctl[0]
+    g.totals = b.totals;
+    g.nblk = nblk;
+    g.gcells = (int64_t)ng * ng * ng;
+    const unsigned nall = (unsigned)nblk * (unsigned)b.nbox;
+    k_deposit_owner<<<nall, OWN_NT, 0, s>>>(g);
+    // the blocks handed over (none, mostly: the workgroups return at once); four
+    // workgroups per CU fill the chip at 32 KB of LDS each
+    k_deposit_owner64<<<nall < 1024u ? nall : 1024u, OWN_NT, 0, s>>>(g);
 }
 void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
     if (!n) return;

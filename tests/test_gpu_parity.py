@@ -113,6 +113,7 @@ def test_deposit_dense_cells_take_the_wide_route(hg, orc, np_):
         assert np.array_equal(m.get_density(), ref)
         assert m.get_tunable("deposit_wide") > 0
         assert m.get_tunable("deposit_deferred") == 0
+        assert m.get_tunable("deposit_owner_redo") > 0  # 64-bit chunks: the 64-bit owner pass
         m.set_tunable("deposit_arena", 100 * (m.N // 4096))  # room for part of the windows only
         m.cic()
         assert np.array_equal(m.get_density(), ref)
@@ -131,6 +132,50 @@ def test_deposit_dense_cells_take_the_wide_route(hg, orc, np_):
             for a, b in zip(m.get_particles_split(), m2.get_particles_split()):
                 assert np.array_equal(a, b)
 
+
+def test_deposit_owner_sum_wraps_32_bits(hg, orc):
+    """The owner pass sums chunks in 32-bit LDS cells and hands a block to the 64-bit
+    owner pass when a cell's sum reaches 2^32 units.  Eight Lagrangian tiles each pile
+    208 particles onto one lattice point (3.5e9 units per tile: no tile window holds a
+    cell over 32 bits, so no 64-bit window), 2.8e10 units together: the block is
+    redone in 64 bits and the density is bit-exact against the oracle; the lattice
+    alone needs no redo."""
+    ng = 64
+    case(f"deposit owner 32-bit wrap ng={ng}")
+    rng = np.random.default_rng(5)
+    p = hg.make_params(ng, ng, 2.0 * ng, 50.0, 0.0, 100)
+    with hg.MemoryManager(p) as m:
+        cell, off = lattice_split(rng, ng, ng, 0.3)
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        assert np.array_equal(m.get_density(), orc.cic_split(ng, cell, off))
+        assert m.get_tunable("deposit_owner_redo") == 0
+        q = np.indices((ng,) * 3).reshape(3, -1).T
+        pile = (q[:, 0] % 16 < 13) & (q[:, 1] % 16 == 0) & (q[:, 0] < 32) & (q[:, 1] < 32) & (q[:, 2] < 32)
+        assert pile.sum() == 8 * 13 * 16
+        cell[pile] = 30
+        off[pile] = 0.0
+        m.set_particles_split(cell, off, np.zeros((m.N, 3), np.float32))
+        m.cic()
+        got = m.get_density()
+        assert np.array_equal(got, orc.cic_split(ng, cell, off))
+        assert got.max() >= 2 ** 32
+        assert m.get_tunable("deposit_owner_redo") > 0 and m.get_tunable("deposit_wide") == 0
+        # the same through the step's FP32 density handoff: one step matches the
+        # separate calls bit for bit
+        with hg.MemoryManager(p) as m2:
+            vel = rng.normal(0, 0.3, (m.N, 3)).astype(np.float32)
+            m.set_particles_split(cell, off, vel)
+            m2.set_particles_split(cell, off, vel)
+            co = hg.schedule(p, 2, merge=True)[0]
+            m.step(co)
+            if not co.skip_drift0:
+                m2.update_positions(co.drift0)
+            m2.cic()
+            m2.solve_gradient()
+            m2.kick_drift(co.kick_dt, co.kick_fscal, co.drift1)
+            for a, b in zip(m.get_particles_split(), m2.get_particles_split()):
+                assert np.array_equal(a, b)
 
 def test_deposit_mass_ratio(hg, orc):
     case("deposit np=32 ng=64 mass 8")
