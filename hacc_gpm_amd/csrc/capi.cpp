@@ -60,7 +60,7 @@ struct hgpm_ctx {
     unsigned long long arena_alloc = 0, arena_cap = 0; // cells allocated / usable
     void* dslots = nullptr;
     unsigned* dcount = nullptr;
-    unsigned* dredo = nullptr; // owner blocks listed for the 64-bit owner pass (after dcount, one allocation)
+    unsigned char* dwide = nullptr; // owner blocks' marks for the 64-bit owner pass (after dcount, one allocation)
     unsigned* deferred = nullptr;
     unsigned* wide = nullptr;
     unsigned short* wide_slots = nullptr;
@@ -94,6 +94,10 @@ struct hgpm_ctx {
     int var_cols = 2, var_xs = 2;
     int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
     int dep_ppt = 8;    // deposit tile: a whole 16^3 Lagrangian tile (8) or half (4), tunable "deposit_tile"
+    // owner-pass routing: the last count of blocks that needed 64-bit sums the host has
+    // (pinned, copied back after each deposit without waiting) and the current choice
+    unsigned long long* need_host = nullptr;
+    bool owner_all64 = false;
 };
 
 namespace {
@@ -235,7 +239,7 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
     b.cap = c->arena_cap;
     b.slots = c->dslots;
     b.count = c->dcount;
-    b.redo = c->dredo;
+    b.wide_blk = c->dwide;
     b.nslots = c->nslots;
     b.deferred = c->deferred;
     b.wide = c->wide;
@@ -272,7 +276,17 @@ int f32_deposit(hgpm_ctx* c, bool fused = false) {
     }
     {
         Timed t2(c, "deposit_owner");
-        launch_deposit_owner(c->ng, (long long*)c->B, fused ? c->psi : nullptr, b, c->st);
+        // every block in 64-bit sums when the last count the host has shows more than
+        // 1/8 of the blocks needing them (back below 1/16).  The count comes back without
+        // waiting, so the choice may lag a few deposits: only the time depends on it
+        const unsigned long long need = *(volatile unsigned long long*)c->need_host;
+        const unsigned long long nall = (unsigned long long)(c->ng / 16) * (c->ng / 16) * (c->ng / 16) * c->nbox;
+        if (!c->owner_all64 && need * 8 > nall) c->owner_all64 = true;
+        else if (c->owner_all64 && need * 16 < nall) c->owner_all64 = false;
+        launch_deposit_owner(c->ng, (long long*)c->B, fused ? c->psi : nullptr, b, c->st, c->owner_all64);
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(c->st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone)
+            HIPCHK(c, hipMemcpyAsync(c->need_host, c->ctl, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
     }
     {
         Timed t3(c, "deposit_deferred");
@@ -455,6 +469,12 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
         set_global_error(m);
         return code;
     };
+    if (hipHostMalloc((void**)&c->need_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        c->need_host = nullptr;
+        c->err = "hipHostMalloc failed";
+        return bail(HGPM_ERR_HIP);
+    }
+    *c->need_host = 0ull;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(HGPM_ERR_HIP);
@@ -490,7 +510,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
             c->arena_cap = c->arena_alloc;
             if ((rc = dalloc(c, &c->arena, sizeof(unsigned) * c->arena_alloc, "deposit arena")) ||
                 (rc = dalloc(c, (char**)&c->dslots, 16 * HGPM_DEP_SLOTS * nblk, "deposit slots")) ||
-                (rc = dalloc(c, &c->dcount, sizeof(unsigned) * 2 * nblk, "deposit counts and redo list")) ||
+                (rc = dalloc(c, (char**)&c->dcount, (sizeof(unsigned) + 1) * nblk, "deposit counts and marks")) ||
                 (rc = dalloc(c, &c->deferred, sizeof(unsigned) * ntiles, "deposit deferred list")) ||
                 (rc = dalloc(c, &c->wide, sizeof(unsigned) * ntiles, "deposit 64-bit list")) ||
                 (rc = dalloc(c, &c->wide_slots, sizeof(unsigned short) * HGPM_DEP_WKEEP * ntiles,
@@ -498,9 +518,9 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
                 (rc = dalloc(c, &c->tile_total, sizeof(unsigned long long) * ntiles, "deposit tile totals")) ||
                 (rc = dalloc(c, (char**)&c->boxes, 32 * ntiles, "gather boxes")))
                 return bail(rc);
-            c->dredo = c->dcount + nblk;
+            c->dwide = (unsigned char*)(c->dcount + nblk);
             // particles start at cell 0, offset 0 (valid if never set; a step refuses them)
-            if (hipMemset(c->dcount, 0, sizeof(unsigned) * nblk) != hipSuccess ||
+            if (hipMemset(c->dcount, 0, (sizeof(unsigned) + 1) * nblk) != hipSuccess ||
                 hipMemset(c->pos, 0, sizeof(PosPackH) * NB) != hipSuccess ||
                 hipMemset(c->vel, 0, sizeof(float) * 3 * NB) != hipSuccess) {
                 c->err = "hipMemset failed";
@@ -576,6 +596,7 @@ int hgpm_destroy(hgpm_ctx* c) {
     for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
     for (auto e : c->evpool) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
+    if (c->need_host) (void)hipHostFree(c->need_host);
     delete c;
     return HGPM_OK;
 }
@@ -1597,7 +1618,7 @@ size_t batch_bytes(const hgpm_params& p, int nstore, int nwave) {
     size_t b = (sizeof(PosPackH) + 3 * sizeof(float)) * N * nstore;     // particles
     b += (2 * sizeof(float2) + sizeof(float)) * G * nwave;              // A, B, psi
     b += 6 * sizeof(unsigned) * G * nwave;                              // arena
-    b += (16 * HGPM_DEP_SLOTS + 2 * sizeof(unsigned)) * nblk;           // slots, counts, redo list
+    b += (16 * HGPM_DEP_SLOTS + sizeof(unsigned) + 1) * nblk;           // slots, counts, marks
     b += (2 * sizeof(unsigned) + sizeof(unsigned long long) + 32 + 2 * HGPM_DEP_WKEEP) * ntiles; // tile buffers
     b += 64ull << 20;                                                   // tables, P(k), slack
     return b;

@@ -63,10 +63,13 @@ void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf
 //              mass units stores its window again with 64-bit cells);
 //   slots    : HGPM_DEP_SLOTS chunk descriptors (16 B) per 16^3 owner block;
 //   count    : chunks listed per owner block, zero between deposits;
-//   redo     : one entry per owner block: the blocks the 32-bit owner pass hands to the
-//              64-bit one (a 64-bit chunk, or a cell sum of 2^32 units or more);
+//   wide_blk : one byte per owner block, kept across deposits (zero at creation): bit 1
+//              = handed from the 32-bit owner pass to the 64-bit one in this deposit (a
+//              64-bit chunk, or a cell sum of 2^32 units or more), bit 0 = summed by
+//              the 64-bit pass before (later deposits send it there at once);
 //   deferred : one entry per tile;
-//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] entries of redo,
+//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] blocks handed to
+//              the 64-bit owner pass,
 //              [1] deferred tiles, of
 //              them [3] for a full slot list or sub-arena (the others had too many
 //              chunks); [2] tiles stored with 64-bit cells; [4 + s] cells reserved in
@@ -83,7 +86,7 @@ struct DepositBuffers {
     unsigned long long cap;
     void* slots;
     unsigned* count;
-    unsigned* redo;
+    unsigned char* wide_blk; // one byte per owner block, kept across deposits
     int nslots;
     unsigned* deferred;
     unsigned* wide; // tiles stored again with 64-bit cells: one entry per tile
@@ -102,7 +105,9 @@ void launch_deposit_windows(const PosPackH* pos, int64_t np3, int ng, float mass
                             hipStream_t s);
 // rhof non-null: if no tile was deferred, write the FP32 density (q - mean) * 2^-24 to
 // rhof instead of the int64 density to rho (needs the totals: launch_deposit_totals first)
-void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s);
+// all64: every block in 64-bit sums (one kernel; the blocks that need them counted in
+// ctl[0]); otherwise the 32-bit pass, which hands blocks to the 64-bit one (counted)
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s, bool all64);
 // totals[box] = sum of the box's tile totals
 void launch_deposit_totals(int64_t np3, const DepositBuffers& b, hipStream_t s);
 void launch_deposit_deferred(const PosPackH* pos, int64_t np3, int ng, float mass, long long* rho,

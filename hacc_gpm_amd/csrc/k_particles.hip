@@ -636,7 +636,13 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_wide(DepWindowArgs g) {
 // eight waves per SIMD) and hands a block to k_deposit_owner64 (int64 LDS cells) when
 // one of its chunks has 64-bit cells or a cell's sum reached 2^32 (its stored cells
 // then sum to less than the values added).  A handed-over block writes nothing and
-// keeps its count; k_deposit_owner64 takes the blocks listed in redo (ctl[0] of them).
+// keeps its count.  wide_blk (one byte per block, kept across deposits): bit 1 = handed
+// over in this deposit (ctl[0] counts them), bit 0 = found to need 64-bit sums by an
+// earlier deposit, which sends the block to the 64-bit pass at once (a dense block
+// stays dense).  When many blocks need 64-bit sums the host runs k_deposit_owner64_all
+// alone (every block in 64 bits, the ones that need it counted in ctl[0]).  The marks
+// and the choice only route blocks (every kernel gives the same bits): a stale one
+// costs time, never results.
 // rhof (may be null): when no tile was deferred (ctl[1] == 0) the block is written as
 // FP32 (q - mean) * 2^-24 to rhof, the form the forward z transform takes, instead of
 // int64 q to rho; mean = totals[box] / ng^3 (the totals of every tile are final).
@@ -644,7 +650,7 @@ struct OwnerArgs {
     const unsigned* __restrict arena;
     const uint4* __restrict slots;
     unsigned* __restrict count;
-    unsigned* __restrict redo;
+    unsigned char* __restrict wide_blk; // see k_deposit_owner64
     int nslots, ng;
     unsigned long long* __restrict rho;
     float* __restrict rhof;
@@ -652,6 +658,8 @@ struct OwnerArgs {
     unsigned long long* __restrict nredo; // = ctl[0]
     const unsigned long long* __restrict totals;
     int nblk;
+    int nall; // owner blocks of every box
+    int count_need; // k_deposit_owner64_all: count the blocks that need 64-bit sums in ctl[0]
     int64_t gcells;
 };
 constexpr int OWN_U32 = 8;  // arena loads in flight per lane, 32-bit pass
@@ -666,6 +674,15 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
     const int box = blk / g.nblk, lb = blk - box * g.nblk;
     const bool f32 = g.rhof != nullptr && g.ctl[1] == 0ull;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (!A64 && g.wide_blk[blk]) {
+        // summed in 64 bits by an earlier deposit (a dense block stays dense): straight
+        // to the 64-bit pass, without a 32-bit attempt
+        if (tid == 0) {
+            g.wide_blk[blk] = 3;
+            atomicAdd(g.nredo, 1ull);
+        }
+        return false;
+    }
     // every slot descriptor is loaded together with the count (the unused ones are
     // ignored), so the two loads cost one memory latency
     uint4 dl = make_uint4(0u, 0u, 0x80000000u, 0u);
@@ -681,7 +698,7 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
     for (int i = tid; i < OB * OB * OB; i += OWN_NT) acc[i] = (ACC)0;
     __syncthreads();
     const int nd = *s_n;
-    bool wrap = false;             // a chunk with 64-bit cells (32-bit pass)
+    bool wrap = false;             // a chunk with 64-bit cells
     unsigned long long added = 0;  // this lane's sum of the values added (32-bit pass)
     for (int d = wv; d < nd; d += OWN_NT / 64) {
         // one descriptor per wave: its words are uniform (scalar registers)
@@ -712,6 +729,7 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
             const int la = ox * OB * OB + (oy + ly) * OB + oz + lz; // LDS cell of x plane 0
             if (A64 && (de.z & 0x40000000u)) {
                 // a window of 64-bit cells (two arena words per cell)
+                wrap = true;
                 for (int x0 = 0; x0 < mx; x0 += U) {
                     unsigned long long v[U];
 #pragma unroll
@@ -751,7 +769,7 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
             }
         }
     }
-    if (!A64 && wrap) *s_redo = 1;
+    if (wrap) *s_redo = 1;
     __syncthreads();
     // thread tid writes (x, y, z) = (k, t / OB, t % OB), t = tid % OB^2, of the block for
     // k = tid / OB^2, tid / OB^2 + KS, ...
@@ -770,27 +788,47 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
         if (lane == 0) atomicAdd(s_dif, dif);
         __syncthreads();
         if (*s_redo || *s_dif) {
-            if (tid == 0) g.redo[atomicAdd(g.nredo, 1ull)] = (unsigned)blk;
+            if (tid == 0) {
+                g.wide_blk[blk] = 2;
+                atomicAdd(g.nredo, 1ull);
+            }
             return false;
         }
     }
     const int bx = lb / (nb * nb), by = (lb / nb) % nb, bz = lb % nb;
     const int64_t at = gidx(g.ng, bx * OB, by * OB + (t >> 4), bz * OB + (t & 15));
     const int64_t plane = (int64_t)g.ng * g.ng;
+    unsigned long long hi = 0; // a cell of 2^32 units or more (64-bit pass)
     if (f32) {
         const double mean = (double)(long long)g.totals[box] / (double)g.gcells;
         float* __restrict rf = g.rhof + (int64_t)box * g.gcells;
 #pragma unroll 4
         for (int k = k0; k < OB; k += KS) {
-            const double q = (double)(long long)(unsigned long long)acc[k * OB * OB + t];
-            rf[at + k * plane] = (float)((q - mean) * (1.0 / 16777216.0));
+            const unsigned long long q64 = (unsigned long long)acc[k * OB * OB + t];
+            hi |= q64 >> 32;
+            rf[at + k * plane] = (float)(((double)(long long)q64 - mean) * (1.0 / 16777216.0));
         }
     } else {
         unsigned long long* __restrict r = g.rho + (int64_t)box * g.gcells;
 #pragma unroll 4
-        for (int k = k0; k < OB; k += KS) r[at + k * plane] = (unsigned long long)acc[k * OB * OB + t];
+        for (int k = k0; k < OB; k += KS) {
+            const unsigned long long q64 = (unsigned long long)acc[k * OB * OB + t];
+            hi |= q64 >> 32;
+            r[at + k * plane] = q64;
+        }
     }
-    if (!A64 && tid == 0) g.count[blk] = 0u;
+    if constexpr (A64) {
+        // mark the block for later deposits: it needs 64-bit sums (a 64-bit chunk or a
+        // cell of 2^32 units or more); with every block in this pass (count_need), count it
+        if (__builtin_amdgcn_ballot_w64(hi != 0ull) && lane == 0) *s_redo = 1;
+        __syncthreads();
+        if (tid == 0) {
+            g.wide_blk[blk] = *s_redo ? 1 : 0;
+            if (g.count_need && *s_redo) atomicAdd(g.nredo, 1ull);
+        }
+    } else {
+        if (tid == 0) g.count[blk] = 0u;
+    }
     return true;
 }
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(OwnerArgs g) {
@@ -800,15 +838,40 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(OwnerArgs g) {
     __shared__ unsigned long long s_dif;
     owner_block<unsigned, OWN_U32>(g, (int)xcd_remap(blockIdx.x, gridDim.x), acc, desc, &s_n, &s_redo, &s_dif);
 }
+// Every block in 64-bit cells, one workgroup per block (a deposit with many dense
+// blocks, late in a run: the 32-bit attempt would be wasted on them).
+__global__ __launch_bounds__(OWN_NT) void k_deposit_owner64_all(OwnerArgs g) {
+    __shared__ unsigned long long acc[OB * OB * OB];
+    __shared__ uint4 desc[HGPM_DEP_SLOTS];
+    __shared__ int s_n, s_redo;
+    __shared__ unsigned long long s_dif;
+    owner_block<unsigned long long, OWN_U64>(g, (int)xcd_remap(blockIdx.x, gridDim.x), acc, desc, &s_n, &s_redo, &s_dif);
+}
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner64(OwnerArgs g) {
     __shared__ unsigned long long acc[OB * OB * OB];
     __shared__ uint4 desc[HGPM_DEP_SLOTS];
     __shared__ int s_n, s_redo;
     __shared__ unsigned long long s_dif;
-    const unsigned long long n = *g.nredo;
-    for (unsigned long long q = blockIdx.x; q < n; q += gridDim.x) {
-        owner_block<unsigned long long, OWN_U64>(g, (int)g.redo[q], acc, desc, &s_n, &s_redo, &s_dif);
-        __syncthreads();
+    __shared__ unsigned long long s_mask[OWN_NT / 64];
+    // a workgroup scans OWN_NT consecutive blocks (one mark per thread) and sums the
+    // handed-over ones in turn: neighbouring blocks, which share the arena lines of
+    // their tiles' windows, stay on one XCD, and a deposit with none handed over costs
+    // one mark load per block
+    const int base = (int)xcd_remap(blockIdx.x, gridDim.x) * OWN_NT;
+    const int tid = threadIdx.x;
+    const bool f = base + tid < g.nall && (g.wide_blk[base + tid] & 2);
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(f);
+    if ((tid & 63) == 0) s_mask[tid >> 6] = m;
+    __syncthreads();
+#pragma unroll 1
+    for (int w = 0; w < OWN_NT / 64; w++) {
+        unsigned long long mm = s_mask[w];
+        while (mm) {
+            const int bit = __builtin_ctzll(mm);
+            mm &= mm - 1;
+            owner_block<unsigned long long, OWN_U64>(g, base + 64 * w + bit, acc, desc, &s_n, &s_redo, &s_dif);
+            __syncthreads();
+        }
     }
 }
 
@@ -1386,14 +1449,14 @@ void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, 
     if (b.ppt == 4) run(std::integral_constant<int, 4>{});
     else run(std::integral_constant<int, 8>{});
 }
-void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s, bool all64) {
     const unsigned nb = (unsigned)(ng / OB);
     const int nblk = (int)(nb * nb * nb);
     OwnerArgs g;
     g.arena = b.arena;
     g.slots = (const uint4*)b.slots;
     g.count = b.count;
-    g.redo = b.redo;
+    g.wide_blk = b.wide_blk;
     g.nslots = b.nslots;
     g.ng = ng;
     g.rho = (unsigned long long*)rho;
@@ -1404,10 +1467,15 @@ void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuff
     g.nblk = nblk;
     g.gcells = (int64_t)ng * ng * ng;
     const unsigned nall = (unsigned)nblk * (unsigned)b.nbox;
+    g.nall = (int)nall;
+    g.count_need = all64 ? 1 : 0;
+    if (all64) {
+        k_deposit_owner64_all<<<nall, OWN_NT, 0, s>>>(g);
+        return;
+    }
     k_deposit_owner<<<nall, OWN_NT, 0, s>>>(g);
-    // the blocks handed over (none, mostly: the workgroups return at once); four
-    // workgroups per CU fill the chip at 32 KB of LDS each
-    k_deposit_owner64<<<nall < 1024u ? nall : 1024u, OWN_NT, 0, s>>>(g);
+    // the blocks handed over (none, mostly), OWN_NT blocks' marks per workgroup
+    k_deposit_owner64<<<(nall + OWN_NT - 1) / OWN_NT, OWN_NT, 0, s>>>(g);
 }
 void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
