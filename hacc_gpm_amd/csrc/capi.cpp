@@ -96,8 +96,6 @@ struct hgpm_ctx {
     int dep_ppt = 8;    // deposit tile: a whole 16^3 Lagrangian tile (8) or half (4), tunable "deposit_tile"
     // owner-pass routing: the last count of blocks that needed 64-bit sums the host has
     // (pinned, copied back after each deposit without waiting) and the current choice
-    unsigned long long* need_host = nullptr;
-    bool owner_all64 = false;
 };
 
 namespace {
@@ -264,7 +262,7 @@ DepositBuffers dep_buffers(hgpm_ctx* c) {
 int f32_deposit(hgpm_ctx* c, bool fused = false) {
     const DepositBuffers b = dep_buffers(c);
     Timed t(c, "deposit");
-    launch_zero_u64(c->ctl, HGPM_DEP_CTL, c->st);
+    launch_deposit_begin(c->ctl, (unsigned long long)(c->ng / 16) * (c->ng / 16) * (c->ng / 16) * c->nbox, c->st);
     {
         Timed t1(c, "deposit_windows");
         launch_deposit_windows(c->pos, c->N * c->nbox, c->ng, ref_mass(c), b, c->st);
@@ -276,17 +274,7 @@ int f32_deposit(hgpm_ctx* c, bool fused = false) {
     }
     {
         Timed t2(c, "deposit_owner");
-        // every block in 64-bit sums when the last count the host has shows more than
-        // 1/8 of the blocks needing them (back below 1/16).  The count comes back without
-        // waiting, so the choice may lag a few deposits: only the time depends on it
-        const unsigned long long need = *(volatile unsigned long long*)c->need_host;
-        const unsigned long long nall = (unsigned long long)(c->ng / 16) * (c->ng / 16) * (c->ng / 16) * c->nbox;
-        if (!c->owner_all64 && need * 8 > nall) c->owner_all64 = true;
-        else if (c->owner_all64 && need * 16 < nall) c->owner_all64 = false;
-        launch_deposit_owner(c->ng, (long long*)c->B, fused ? c->psi : nullptr, b, c->st, c->owner_all64);
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(c->st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone)
-            HIPCHK(c, hipMemcpyAsync(c->need_host, c->ctl, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+        launch_deposit_owner(c->ng, (long long*)c->B, fused ? c->psi : nullptr, b, c->st);
     }
     {
         Timed t3(c, "deposit_deferred");
@@ -469,12 +457,6 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
         set_global_error(m);
         return code;
     };
-    if (hipHostMalloc((void**)&c->need_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
-        c->need_host = nullptr;
-        c->err = "hipHostMalloc failed";
-        return bail(HGPM_ERR_HIP);
-    }
-    *c->need_host = 0ull;
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(HGPM_ERR_HIP);
@@ -496,7 +478,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
             (rc = dalloc(c, &c->A, sizeof(float2) * GB, "grid A")) ||
             (rc = dalloc(c, &c->B, sizeof(float2) * GB, "grid B")) ||
             (rc = dalloc(c, &c->psi, sizeof(float) * GB, "potential")) ||
-            (rc = dalloc(c, &c->ctl, HGPM_DEP_CTL * sizeof(unsigned long long), "deposit control")) ||
+            (rc = dalloc(c, &c->ctl, (HGPM_DEP_CTL + 1) * sizeof(unsigned long long), "deposit control")) ||
             (rc = dalloc(c, &c->totals, sizeof(unsigned long long) * nbox, "deposit totals")))
             return bail(rc);
         c->total = c->totals;
@@ -521,6 +503,7 @@ int ctx_create(const hgpm_params* p, int nbox, hgpm_ctx** out, int nwave = 0) {
             c->dwide = (unsigned char*)(c->dcount + nblk);
             // particles start at cell 0, offset 0 (valid if never set; a step refuses them)
             if (hipMemset(c->dcount, 0, (sizeof(unsigned) + 1) * nblk) != hipSuccess ||
+                hipMemset(c->ctl, 0, (HGPM_DEP_CTL + 1) * sizeof(unsigned long long)) != hipSuccess ||
                 hipMemset(c->pos, 0, sizeof(PosPackH) * NB) != hipSuccess ||
                 hipMemset(c->vel, 0, sizeof(float) * 3 * NB) != hipSuccess) {
                 c->err = "hipMemset failed";
@@ -596,7 +579,6 @@ int hgpm_destroy(hgpm_ctx* c) {
     for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
     for (auto e : c->evpool) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
-    if (c->need_host) (void)hipHostFree(c->need_host);
     delete c;
     return HGPM_OK;
 }

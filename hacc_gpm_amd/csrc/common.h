@@ -58,11 +58,13 @@ __device__ __forceinline__ int pk_max16(int a, int b) {
     return __builtin_bit_cast(int, __builtin_elementwise_max(__builtin_bit_cast(hgpm_s16x2, a),
                                                              __builtin_bit_cast(hgpm_s16x2, b)));
 }
+// (the four row-internal steps read a lane of the same row for every lane, so no lane
+// keeps an old value there and a plain DPP move serves)
 __device__ __forceinline__ int wave_max_pk16(int v) {
-    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
-    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
-    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
-    v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+    v = pk_max16(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = pk_max16(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = pk_max16(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = pk_max16(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));
     v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
     v = pk_max16(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
     return __builtin_amdgcn_readlane(v, 63);
@@ -121,6 +123,17 @@ __host__ __device__ __forceinline__ unsigned fx_drift(unsigned X, float v, float
     if (s < 0) s += span;
     else if (s >= span) s -= span;
     return (unsigned)s;
+}
+// fx_drift for a power-of-two ng in 32-bit integers, same bits: ng 2^22 divides 2^32,
+// so the sum reduced modulo 2^32 has the same residue modulo ng 2^22.
+// span_mask = ng 2^22 - 1 (modulo 2^32).
+__host__ __device__ __forceinline__ unsigned fx_drift_p2(unsigned X, float v, float pf, int ng, unsigned span_mask) {
+    float t = v * pf;
+    if (!(fabsf(t) < (float)ng)) t = t - (float)ng * truncf(t / (float)ng);
+    const float ti = truncf(t);
+    const unsigned d = ((unsigned)(int)ti << HGPM_FRAC_BITS) +
+                       (unsigned)(int)rintf((t - ti) * (float)(1u << HGPM_FRAC_BITS));
+    return (X + d) & span_mask;
 }
 
 #define HGPM_WAVE 64

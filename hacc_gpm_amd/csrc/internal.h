@@ -68,12 +68,14 @@ void launch_drift(PosPackH* pos, const float* vel, int64_t np3, int ng, float pf
 //              64-bit chunk, or a cell sum of 2^32 units or more), bit 0 = summed by
 //              the 64-bit pass before (later deposits send it there at once);
 //   deferred : one entry per tile;
-//   ctl      : HGPM_DEP_CTL words, cleared before each deposit: [0] blocks handed to
-//              the 64-bit owner pass,
-//              [1] deferred tiles, of
+//   ctl      : HGPM_DEP_CTL + 1 words.  [0 .. HGPM_DEP_CTL) are cleared at the start of
+//              each deposit (launch_deposit_begin): [0] blocks that went to the 64-bit
+//              owner pass (handed over, or all of them that needed it in the all-64-bit
+//              route), [1] deferred tiles, of
 //              them [3] for a full slot list or sub-arena (the others had too many
 //              chunks); [2] tiles stored with 64-bit cells; [4 + s] cells reserved in
-//              sub-arena s;
+//              sub-arena s; [HGPM_DEP_CTL], kept across deposits (zero at creation):
+//              the owner pass's route, 1 = every block in 64-bit sums;
 //   tile_total: deposited mass per tile.
 // A deposit tile is DEP_NT * ppt consecutive particles: a whole 16^3 Lagrangian tile
 // (ppt 8) or half of one (ppt 4); per-tile buffers are sized for half tiles.
@@ -103,11 +105,16 @@ struct DepositBuffers {
 // cell over 2^32 - 1 mass units).  ctl must be cleared and the chunk counts zero.
 void launch_deposit_windows(const PosPackH* pos, int64_t np3, int ng, float mass, const DepositBuffers& b,
                             hipStream_t s);
+// Start of a deposit: the owner pass's route from the previous deposit's ctl[0] (every
+// block in 64-bit sums when more than 1/8 of the nall owner blocks needed them, back to
+// the 32-bit pass below 1/16), then the per-deposit control words cleared.
+void launch_deposit_begin(unsigned long long* ctl, unsigned long long nall, hipStream_t s);
 // rhof non-null: if no tile was deferred, write the FP32 density (q - mean) * 2^-24 to
 // rhof instead of the int64 density to rho (needs the totals: launch_deposit_totals first)
-// all64: every block in 64-bit sums (one kernel; the blocks that need them counted in
-// ctl[0]); otherwise the 32-bit pass, which hands blocks to the 64-bit one (counted)
-void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s, bool all64);
+// The route in ctl[HGPM_DEP_CTL] picks, on the device, either every block in 64-bit sums
+// (one kernel; the blocks that need them counted in ctl[0]) or the 32-bit pass, which
+// hands blocks to the 64-bit one (counted); the kernels of the other route exit at once.
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s);
 // totals[box] = sum of the box's tile totals
 void launch_deposit_totals(int64_t np3, const DepositBuffers& b, hipStream_t s);
 void launch_deposit_deferred(const PosPackH* pos, int64_t np3, int ng, float mass, long long* rho,

@@ -128,15 +128,34 @@ __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int6
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int d = 0; d < 3; d++) { mn[h][d] = INT_MAX; mx[h][d] = INT_MIN; }
+    if ((ng & (ng - 1)) == 0) {
+        // power-of-two ng: the relative cell is the 32-bit difference of the coordinates
+        // (the origin's fraction cleared) with its log2(ng) cell bits sign-extended, the
+        // residue modulo ng in [-ng/2, ng/2) that wrap_rel forms
+        const int lg = __builtin_ctz(ng);
+        const unsigned ox = (unsigned)r0.cx << HGPM_FRAC_BITS, oy = (unsigned)r0.cy << HGPM_FRAC_BITS,
+                       oz = (unsigned)r0.cz << HGPM_FRAC_BITS;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const PosPack q = pos[p0 + tid + k * DEP_NT]; // tiles are full: n is a multiple of DEP_TILE
+            R.rx[k] = __builtin_amdgcn_sbfe((int)(q.x - ox), HGPM_FRAC_BITS, lg);
+            R.ry[k] = __builtin_amdgcn_sbfe((int)(q.y - oy), HGPM_FRAC_BITS, lg);
+            R.rz[k] = __builtin_amdgcn_sbfe((int)(q.z - oz), HGPM_FRAC_BITS, lg);
+            R.fx[k] = fx_off(q.x); R.fy[k] = fx_off(q.y); R.fz[k] = fx_off(q.z);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const PosSplit p = unpack(pos[p0 + tid + k * DEP_NT]);
+            R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
+            R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
+            R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
+            R.fx[k] = p.fx; R.fy[k] = p.fy; R.fz[k] = p.fz;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         const int h = k < PPT / 2 ? 0 : 1;
-        const int64_t j = p0 + tid + k * DEP_NT;
-        const PosSplit p = unpack(pos[j]); // tiles are full: n is a multiple of DEP_TILE
-        R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
-        R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
-        R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
-        R.fx[k] = p.fx; R.fy[k] = p.fy; R.fz[k] = p.fz;
         mn[h][0] = min(mn[h][0], R.rx[k]); mx[h][0] = max(mx[h][0], R.rx[k]);
         mn[h][1] = min(mn[h][1], R.ry[k]); mx[h][1] = max(mx[h][1], R.ry[k]);
         mn[h][2] = min(mn[h][2], R.rz[k]); mx[h][2] = max(mx[h][2], R.rz[k]);
@@ -191,20 +210,29 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
                                                              const int e[3], int sx0, int nx, int sy0, int ny,
                                                              float mass) {
     unsigned long long dep = 0;
+    constexpr int CS = (int)sizeof(W);
     const int ez = e[2], sxs = ny * ez;
     const int lx = (int)(threadIdx.x >> 1) & 1, ly = (int)threadIdx.x & 1;
-    const int ox_[2] = {lx * sxs, (1 - lx) * sxs}, oy_[2] = {ly * ez, (1 - ly) * ez};
+    // Byte offsets in the window: a particle's first corner row, (x ^ lx, y ^ ly) of its
+    // base cell, lies at Sx rx + Sy ry + CS rz + c0 (24-bit products, full rate: the
+    // relative cells are below 2^10 and the strides below 2^17 in magnitude), the other
+    // three at the lane's row steps from it.
+    const int Sx = CS * sxs, Sy = CS * ez;
+    const int c0 = lx * Sx + ly * Sy - (Sx * (lo[0] + sx0) + Sy * (lo[1] + sy0) + CS * lo[2]);
+    const int dxr = lx ? -Sx : Sx, dyr = ly ? -Sy : Sy;
+    const int rstep[2][2] = {{0, dyr}, {dxr, dxr + dyr}};
+    char* const wb = (char*)win;
 #pragma unroll
     for (int k = 0; k < PPT; k++) {
         // the weights are slab-invariant; an opaque copy of the offsets stops the
         // compiler from hoisting all 64 of them out of the slab loop (register spill)
         float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
         if (!ALL) asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
-        const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0, bz = R.rz[k] - lo[2];
+        const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0; // slab bounds checks
         const float gx[2] = {1.0f - ox, ox};
         const float gy[2] = {1.0f - oy, oy};
         const float gz[2] = {1.0f - oz, oz};
-        const int b0 = bx * sxs + by * ez + bz;
+        const int b0 = __mul24(R.rx[k], Sx) + __mul24(R.ry[k], Sy) + CS * R.rz[k] + c0;
         // one particle's 8 weights sum to about mass * 2^24 < 2^31 when SMALL
         DepW<SMALL> pd = 0;
         // step (jx, jy) adds the corner row (jx ^ lx, jy ^ ly): lanes whose particles
@@ -221,7 +249,7 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
             for (int jy = 0; jy < 2; jy++) {
                 const int y = jy ^ ly;
                 if (!ALL && (by + y < 0 || by + y >= ny)) continue;
-                W* row = &win[b0 + ox_[jx] + oy_[jy]];
+                W* row = (W*)(wb + (b0 + rstep[jx][jy]));
                 const DepW<SMALL> q0 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[0], mass);
                 const DepW<SMALL> q1 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[1], mass);
                 pd += q0 + q1;
@@ -639,10 +667,11 @@ __global__ __launch_bounds__(DEP_NT) void k_deposit_wide(DepWindowArgs g) {
 // keeps its count.  wide_blk (one byte per block, kept across deposits): bit 1 = handed
 // over in this deposit (ctl[0] counts them), bit 0 = found to need 64-bit sums by an
 // earlier deposit, which sends the block to the 64-bit pass at once (a dense block
-// stays dense).  When many blocks need 64-bit sums the host runs k_deposit_owner64_all
-// alone (every block in 64 bits, the ones that need it counted in ctl[0]).  The marks
-// and the choice only route blocks (every kernel gives the same bits): a stale one
-// costs time, never results.
+// stays dense).  When many blocks needed 64-bit sums in the previous deposit, the route
+// word ctl[HGPM_DEP_CTL] (set by k_deposit_begin) sends every block to
+// k_deposit_owner64_all instead (the ones that need it counted in ctl[0]), and the
+// 32-bit kernels exit at once.  The marks and the route only route blocks (every kernel
+// gives the same bits): a stale one costs time, never results.
 // rhof (may be null): when no tile was deferred (ctl[1] == 0) the block is written as
 // FP32 (q - mean) * 2^-24 to rhof, the form the forward z transform takes, instead of
 // int64 q to rho; mean = totals[box] / ng^3 (the totals of every tile are final).
@@ -725,20 +754,28 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
             const int pl = lane + 64 * j;
             const bool pv = pl < myz;
             const int ly = (int)(((float)pl + 0.5f) * rmz), lz = pl - ly * mz;
-            const unsigned po = pv ? (unsigned)(ly * e2 + lz) : 0u; // arena offset in the plane
+            unsigned po = pv ? (unsigned)(ly * e2 + lz) : 0u; // arena offset in the plane
+            // (opaque: the address of plane x stays x * e12 + po, a scalar product and one
+            // add, instead of being refactored into a multiply per load)
+            asm volatile("" : "+v"(po));
             const int la = ox * OB * OB + (oy + ly) * OB + oz + lz; // LDS cell of x plane 0
+            // lanes past the plane sit the chunk out; inside, every condition is uniform
+            // (scalar branches): the values are added unconditionally (zeros included),
+            // which costs less than masking each add
+            if (!pv) continue;
             if (A64 && (de.z & 0x40000000u)) {
                 // a window of 64-bit cells (two arena words per cell)
                 wrap = true;
                 for (int x0 = 0; x0 < mx; x0 += U) {
                     unsigned long long v[U];
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (x0 + u < mx)
-                            v[u] = *(const unsigned long long*)((const char*)ch + ((unsigned)(x0 + u) * e12 + po) * 8u);
+                    for (int u = 0; u < U; u++) {
+                        const unsigned xu = (unsigned)min(x0 + u, mx - 1);
+                        v[u] = *(const unsigned long long*)((const char*)ch + (xu * e12 + po) * 8u);
+                    }
 #pragma unroll
                     for (int u = 0; u < U; u++)
-                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (ACC)v[u]);
+                        if (x0 + u < mx) atomicAdd(&acc[la + (x0 + u) * OB * OB], (ACC)v[u]);
                 }
                 continue;
             }
@@ -746,24 +783,25 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
                 unsigned v[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    // every slot loads (planes past the chunk's last load that one again,
-                    // lanes past the plane its first cell) so that the U loads issue
-                    // together.  A chunk lies in a window of at most DEP_MAXCH owner
-                    // blocks: byte offsets from its first cell fit 32 bits (also for
-                    // 64-bit cells)
+                    // every slot loads (planes past the chunk's last load that one again)
+                    // so that the U loads issue together.  A chunk lies in a window of at
+                    // most DEP_MAXCH owner blocks: byte offsets from its first cell fit 32
+                    // bits (also for 64-bit cells)
                     const unsigned xu = (unsigned)min(x0 + u, mx - 1);
                     v[u] = *(const unsigned*)((const char*)ch + (xu * e12 + po) * 4u);
                 }
-                if constexpr (A64) {
+                if (x0 + U <= mx) {
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (x0 + u < mx && pv && v[u]) atomicAdd(&acc[la + (x0 + u) * OB * OB], (unsigned long long)v[u]);
+                    for (int u = 0; u < U; u++) {
+                        atomicAdd(&acc[la + (x0 + u) * OB * OB], (ACC)v[u]);
+                        if constexpr (!A64) added += v[u];
+                    }
                 } else {
 #pragma unroll
                     for (int u = 0; u < U; u++)
-                        if (x0 + u < mx && pv && v[u]) {
-                            atomicAdd(&acc[la + (x0 + u) * OB * OB], v[u]);
-                            added += v[u];
+                        if (x0 + u < mx) {
+                            atomicAdd(&acc[la + (x0 + u) * OB * OB], (ACC)v[u]);
+                            if constexpr (!A64) added += v[u];
                         }
                 }
             }
@@ -831,7 +869,20 @@ __device__ __forceinline__ bool owner_block(const OwnerArgs& g, int blk, ACC* ac
     }
     return true;
 }
+// ctl[0] of the previous deposit -> the route; then the control words of this one cleared
+__global__ __launch_bounds__(64) void k_deposit_begin(unsigned long long* __restrict ctl, unsigned long long nall) {
+    if (threadIdx.x == 0) {
+        const unsigned long long need = ctl[0];
+        unsigned long long r = ctl[HGPM_DEP_CTL];
+        if (!r && need * 8 > nall) r = 1;
+        else if (r && need * 16 < nall) r = 0;
+        ctl[HGPM_DEP_CTL] = r;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < HGPM_DEP_CTL; i += blockDim.x) ctl[i] = 0ull;
+}
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(OwnerArgs g) {
+    if (g.ctl[HGPM_DEP_CTL]) return; // every block in k_deposit_owner64_all
     __shared__ unsigned acc[OB * OB * OB];
     __shared__ uint4 desc[HGPM_DEP_SLOTS];
     __shared__ int s_n, s_redo;
@@ -841,6 +892,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner(OwnerArgs g) {
 // Every block in 64-bit cells, one workgroup per block (a deposit with many dense
 // blocks, late in a run: the 32-bit attempt would be wasted on them).
 __global__ __launch_bounds__(OWN_NT) void k_deposit_owner64_all(OwnerArgs g) {
+    if (!g.ctl[HGPM_DEP_CTL]) return; // the 32-bit route
     __shared__ unsigned long long acc[OB * OB * OB];
     __shared__ uint4 desc[HGPM_DEP_SLOTS];
     __shared__ int s_n, s_redo;
@@ -853,6 +905,7 @@ __global__ __launch_bounds__(OWN_NT) void k_deposit_owner64(OwnerArgs g) {
     __shared__ int s_n, s_redo;
     __shared__ unsigned long long s_dif;
     __shared__ unsigned long long s_mask[OWN_NT / 64];
+    if (g.ctl[HGPM_DEP_CTL]) return; // every block in k_deposit_owner64_all
     // a workgroup scans OWN_NT consecutive blocks (one mark per thread) and sums the
     // handed-over ones in turn: neighbouring blocks, which share the arena lines of
     // their tiles' windows, stay on one XCD, and a deposit with none handed over costs
@@ -952,7 +1005,9 @@ constexpr int GT_SPARSE = 4;               // staged cells per particle above wh
 
 // (registers for the two workgroups per CU the LDS budget allows: 4 waves per SIMD at
 // 512 threads, 2 at 256)
-template <int GT_NT, int GT_PPT, int GMAX, bool BOXED>
+// P2: ng is a power of two (cell offsets as bit fields of coordinate differences, the
+// drift in 32-bit integers: fx_drift_p2)
+template <int GT_NT, int GT_PPT, int GMAX, bool BOXED, bool P2>
 __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __restrict pos, float* __restrict vel, int64_t n,
                                                        int ng, const float* __restrict psi, double deltaT,
                                                        double fscal, int do_drift, float pf,
@@ -967,6 +1022,8 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     const int64_t p0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * GT_TILE;
     // the kick factor deltaT * fscal (icic_pm.cu:110) rounded once to FP32
     const float kc = (float)(deltaT * fscal);
+    const int lg = P2 ? __builtin_ctz(ng) : 0;
+    const unsigned span_mask = ((unsigned)ng << HGPM_FRAC_BITS) - 1u; // mod 2^32: ng = 1024 gives 2^32 - 1
     // the potential of this tile's box (a batch holds several boxes of the same shape)
     psi += (p0 / npb) * gcells;
     // BOXED: the deposit of these positions stored the box of each half deposit tile
@@ -1154,17 +1211,26 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 // particle of the tile lands at >= nbx (its offset is in [-sx0, b - sx0)
                 // with sx0 + nbx <= b <= ng); likewise in y, and in z every particle is
                 // inside [0, b[2])
-                int lx = p.cx - gxs, ly = p.cy - gys, lz = p.cz - gzs;
-                lx += (lx >> 31) & ng;
-                ly += (ly >> 31) & ng;
-                if ((unsigned)lx >= (unsigned)nbx || (unsigned)ly >= (unsigned)nby) continue;
-                lz += (lz >> 31) & ng;
+                int lx, ly, lz;
+                if constexpr (P2) {
+                    // the low log2(ng) cell bits of the coordinate difference
+                    lx = (int)__builtin_amdgcn_ubfe(X.x - ((unsigned)gxs << HGPM_FRAC_BITS), HGPM_FRAC_BITS, lg);
+                    ly = (int)__builtin_amdgcn_ubfe(X.y - ((unsigned)gys << HGPM_FRAC_BITS), HGPM_FRAC_BITS, lg);
+                    if ((unsigned)lx >= (unsigned)nbx || (unsigned)ly >= (unsigned)nby) continue;
+                    lz = (int)__builtin_amdgcn_ubfe(X.z - ((unsigned)gzs << HGPM_FRAC_BITS), HGPM_FRAC_BITS, lg);
+                } else {
+                    lx = p.cx - gxs; ly = p.cy - gys; lz = p.cz - gzs;
+                    lx += (lx >> 31) & ng;
+                    ly += (ly >> 31) & ng;
+                    if ((unsigned)lx >= (unsigned)nbx || (unsigned)ly >= (unsigned)nby) continue;
+                    lz += (lz >> 31) & ng;
+                }
                 const float gx[2] = {1.0f - p.fx, p.fx};
                 const float gy[2] = {1.0f - p.fy, p.fy};
                 const float gz[2] = {1.0f - p.fz, p.fz};
                 // staged index of the base corner; strides of x and y
                 const int sxs = nys * ezp;
-                const int b0 = ((lx + 1) * nys + (ly + 1)) * ezp + (lz + 1);
+                const int b0 = __mul24(__mul24(lx + 1, nys) + (ly + 1), ezp) + (lz + 1); // < GMAX
                 // the 2x2x2 corners and the 24 neighbours their differences need
                 float q[2][2][2], xm[2][2], xp[2][2], ym[2][2], yp[2][2], zm[2][2], zp[2][2];
 #pragma unroll
@@ -1200,9 +1266,15 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 vel[3 * j + 1] = vy;
                 vel[3 * j + 2] = vz;
                 if (do_drift) {
-                    X.x = fx_drift(X.x, vx, pf, ng);
-                    X.y = fx_drift(X.y, vy, pf, ng);
-                    X.z = fx_drift(X.z, vz, pf, ng);
+                    if constexpr (P2) {
+                        X.x = fx_drift_p2(X.x, vx, pf, ng, span_mask);
+                        X.y = fx_drift_p2(X.y, vy, pf, ng, span_mask);
+                        X.z = fx_drift_p2(X.z, vz, pf, ng, span_mask);
+                    } else {
+                        X.x = fx_drift(X.x, vx, pf, ng);
+                        X.y = fx_drift(X.y, vy, pf, ng);
+                        X.z = fx_drift(X.z, vz, pf, ng);
+                    }
                     pos[j] = X;
                 }
             }
@@ -1449,7 +1521,10 @@ void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, 
     if (b.ppt == 4) run(std::integral_constant<int, 4>{});
     else run(std::integral_constant<int, 8>{});
 }
-void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s, bool all64) {
+void launch_deposit_begin(unsigned long long* ctl, unsigned long long nall, hipStream_t s) {
+    k_deposit_begin<<<1, 64, 0, s>>>(ctl, nall);
+}
+void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuffers& b, hipStream_t s) {
     const unsigned nb = (unsigned)(ng / OB);
     const int nblk = (int)(nb * nb * nb);
     OwnerArgs g;
@@ -1468,14 +1543,12 @@ void launch_deposit_owner(int ng, long long* rho, float* rhof, const DepositBuff
     g.gcells = (int64_t)ng * ng * ng;
     const unsigned nall = (unsigned)nblk * (unsigned)b.nbox;
     g.nall = (int)nall;
-    g.count_need = all64 ? 1 : 0;
-    if (all64) {
-        k_deposit_owner64_all<<<nall, OWN_NT, 0, s>>>(g);
-        return;
-    }
+    g.count_need = 0;
     k_deposit_owner<<<nall, OWN_NT, 0, s>>>(g);
     // the blocks handed over (none, mostly), OWN_NT blocks' marks per workgroup
     k_deposit_owner64<<<(nall + OWN_NT - 1) / OWN_NT, OWN_NT, 0, s>>>(g);
+    g.count_need = 1;
+    k_deposit_owner64_all<<<nall, OWN_NT, 0, s>>>(g);
 }
 void launch_deposit_totals(int64_t n, const DepositBuffers& b, hipStream_t s) {
     if (!n) return;
@@ -1510,12 +1583,19 @@ void gather_launch(PosPackH* pos, float* vel, int64_t n, int ng, const float* ps
                    int do_drift, float pf, const void* boxes, int nbox, hipStream_t s) {
     const unsigned nb = (unsigned)(n / (NT * 4));
     const int64_t npb = n / nbox, gc = (int64_t)ng * ng * ng;
-    if (boxes)
-        k_gather_slab<NT, 4, GT_GMAX, true><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
-                                                              do_drift, pf, (const int4*)boxes, npb, gc);
+    const bool p2 = (ng & (ng - 1)) == 0;
+    if (boxes && p2)
+        k_gather_slab<NT, 4, GT_GMAX, true, true><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
+                                                                    do_drift, pf, (const int4*)boxes, npb, gc);
+    else if (boxes)
+        k_gather_slab<NT, 4, GT_GMAX, true, false><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
+                                                                     do_drift, pf, (const int4*)boxes, npb, gc);
+    else if (p2)
+        k_gather_slab<NT, 4, GT_GMAX, false, true><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
+                                                                     do_drift, pf, nullptr, npb, gc);
     else
-        k_gather_slab<NT, 4, GT_GMAX, false><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT, fscal,
-                                                               do_drift, pf, nullptr, npb, gc);
+        k_gather_slab<NT, 4, GT_GMAX, false, false><<<nb, NT, 0, s>>>((PosPack*)pos, vel, n, ng, psi, deltaT,
+                                                                      fscal, do_drift, pf, nullptr, npb, gc);
 }
 void launch_gather_kick(PosPackH* pos, float* vel, int64_t n, int ng, const float* psi, double deltaT,
                         double fscal, int do_drift, float pf, int variant, const void* boxes, int nbox,
