@@ -1160,10 +1160,18 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                         use[u] = i0 + u * GT_NT < nitems && (!masked || (msk[GT_MASKW + x] >> y & 1ull));
                         if (use[u]) {
                             int gx = bx0 + x, gy = by0 + y;
-                            gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
-                            gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
-                            const float* __restrict rp =
-                                psi + ((unsigned)gx * (unsigned)ng + (unsigned)gy) * (unsigned)ng;
+                            unsigned rowc; // first cell of grid row (gx, gy)
+                            if constexpr (P2) {
+                                gx &= ng - 1;
+                                gy &= ng - 1;
+                                rowc = ((unsigned)gx << lg | (unsigned)gy) << lg;
+                            } else {
+                                gx = gx < 0 ? gx + ng : (gx >= ng ? gx - ng : gx);
+                                gy = gy < 0 ? gy + ng : (gy >= ng ? gy - ng : gy);
+                                // 24-bit products: gx ng + gy < 2^20
+                                rowc = __umul24(__umul24((unsigned)gx, (unsigned)ng) + (unsigned)gy, (unsigned)ng);
+                            }
+                            const float* __restrict rp = psi + rowc;
                             if (zin) {
                                 const int z = wz0 + 4 * sg[u];
 #pragma unroll
