@@ -40,16 +40,20 @@ __global__ __launch_bounds__(BS) void k_drift(PosPack* __restrict pos, const flo
 }
 
 // CIC weight of the corner (x,y,z) in FP32: ((dx*dy)*dz)*mass with dx = f or 1-f,
-// rounded to nearest-even in units of 2^-24.  SMALL (mass < 128, so the scaled weight
-// is below 2^31): round in FP32 and convert exactly through int32, the same value as
-// the int64 conversion at a fraction of its instructions.
+// rounded to nearest-even in units of 2^-24.  The factors come in units of 2^-22 cells
+// (dx 2^22, the integer the fixed-point position holds, and 2^22 - it, both exact):
+// products of scaled factors round exactly as the unscaled ones (a power-of-two scale
+// commutes with rounding while no value is subnormal: w >= 2^-66 when non-zero), so the
+// weight is W = w 2^66 exactly, and W * (mass 2^-42) is the same real number as
+// w * (mass 2^24), rounded once (mass 2^-42 is exact for any mass above 2^-84).
+// SMALL (mass < 128, so the scaled weight is below 2^31): round in FP32 and convert
+// exactly through int32, the same value as the int64 conversion at a fraction of its
+// instructions.
 template <bool SMALL>
 __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, float mass) {
     float w = dx * dy;
     w = w * dz;
-    // (w * mass) * 2^24 in one product: scaling by 2^24 is exact, so w * (mass * 2^24)
-    // rounds to the same value, and where w * mass would be subnormal both round to 0
-    const float q = w * (mass * 16777216.0f);
+    const float q = w * (mass * 0x1p-42f);
     if constexpr (SMALL) return (long long)(int)rintf(q);
     else return (long long)__float2ll_rn(q);
 }
@@ -109,10 +113,12 @@ __device__ __forceinline__ int wrap_cell(int v, int ng) {
 // (periodic-unwrapped) and offsets.
 // PPT particles per thread: DEP_NT * PPT consecutive particles, a whole Lagrangian tile
 // (PPT = 8) or half of one (PPT = 4, an 8 x 16 x 16 lattice block).
+// Offsets in units of 2^-22 cells: the fraction bits of the coordinate as an FP32 integer.
 template <int PPT> struct DepRegs {
     int rx[PPT], ry[PPT], rz[PPT];
     float fx[PPT], fy[PPT], fz[PPT];
 };
+__device__ __forceinline__ float fx_off_units(unsigned X) { return (float)(X & HGPM_FRAC_MASK); }
 
 // Loads the tile starting at p0 and reduces the box of its cells: lo = minimum
 // relative cell, e = extent + 1 (the upper CIC neighbour).  The two halves of the tile
@@ -141,16 +147,16 @@ __device__ __forceinline__ void dep_load_box(const PosPack* __restrict pos, int6
             R.rx[k] = __builtin_amdgcn_sbfe((int)(q.x - ox), HGPM_FRAC_BITS, lg);
             R.ry[k] = __builtin_amdgcn_sbfe((int)(q.y - oy), HGPM_FRAC_BITS, lg);
             R.rz[k] = __builtin_amdgcn_sbfe((int)(q.z - oz), HGPM_FRAC_BITS, lg);
-            R.fx[k] = fx_off(q.x); R.fy[k] = fx_off(q.y); R.fz[k] = fx_off(q.z);
+            R.fx[k] = fx_off_units(q.x); R.fy[k] = fx_off_units(q.y); R.fz[k] = fx_off_units(q.z);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
-            const PosSplit p = unpack(pos[p0 + tid + k * DEP_NT]);
-            R.rx[k] = wrap_rel(p.cx - r0.cx, ng);
-            R.ry[k] = wrap_rel(p.cy - r0.cy, ng);
-            R.rz[k] = wrap_rel(p.cz - r0.cz, ng);
-            R.fx[k] = p.fx; R.fy[k] = p.fy; R.fz[k] = p.fz;
+            const PosPack q = pos[p0 + tid + k * DEP_NT];
+            R.rx[k] = wrap_rel(fx_cell(q.x) - r0.cx, ng);
+            R.ry[k] = wrap_rel(fx_cell(q.y) - r0.cy, ng);
+            R.rz[k] = wrap_rel(fx_cell(q.z) - r0.cz, ng);
+            R.fx[k] = fx_off_units(q.x); R.fy[k] = fx_off_units(q.y); R.fz[k] = fx_off_units(q.z);
         }
     }
 #pragma unroll
@@ -229,9 +235,10 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
         float ox = R.fx[k], oy = R.fy[k], oz = R.fz[k];
         if (!ALL) asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
         const int bx = R.rx[k] - lo[0] - sx0, by = R.ry[k] - lo[1] - sy0; // slab bounds checks
-        const float gx[2] = {1.0f - ox, ox};
-        const float gy[2] = {1.0f - oy, oy};
-        const float gz[2] = {1.0f - oz, oz};
+        constexpr float ONE = (float)(1u << HGPM_FRAC_BITS); // 1 in units of 2^-22
+        const float gx[2] = {ONE - ox, ox};
+        const float gy[2] = {ONE - oy, oy};
+        const float gz[2] = {ONE - oz, oz};
         const int b0 = __mul24(R.rx[k], Sx) + __mul24(R.ry[k], Sy) + CS * R.rz[k] + c0;
         // one particle's 8 weights sum to about mass * 2^24 < 2^31 when SMALL
         DepW<SMALL> pd = 0;
