@@ -1087,8 +1087,13 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
         }
     }
     const int ez = b[2] + GT_HALO;   // staged cells per row, <= ng + 3
-    const int ezp = (ez + 3) & ~3;   // LDS row pitch: whole 4-cell segments, <= ng + 4
-    const int nseg = ezp >> 2;
+    const int ezl = (ez + 3) & ~3;   // loaded per row: whole 4-cell segments, <= ng + 4
+    const int nseg = ezl >> 2;
+    // LDS row pitch: when the whole box fits one slab with it, the smallest pitch >= ezl
+    // that is 16 modulo 32 words, so that the rows a wave's particles read (16 cells of
+    // z in each of 4 rows of the lattice block) start 16 banks apart; else ezl
+    const int p16 = ((ezl + 47) & ~31) - 16;
+    const int ezp = (b[0] + GT_HALO) * (b[1] + GT_HALO) * p16 <= GMAX ? p16 : ezl;
     // slab shape: ty base rows (ty + 3 staged) with 4 (ty + 3) ezp <= GMAX, then tx base
     // planes (tx + 3 staged) with (tx + 3) (ty + 3) ezp <= GMAX; ezp <= ng + 4 <= 1028
     // keeps both >= 1
@@ -1109,7 +1114,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
             // for each base column (x, y) of the slab's particles; a particle with base
             // column (x, y) reads staged rows (x + 1 + u, y .. y + 3) for u = 0, 1 and
             // (x, y + 1 .. y + 2), (x + 3, y + 1 .. y + 2): need[x] collects them.
-            const bool masked = nys <= 64 && nxs <= GT_MASKW && nxs * nys * ezp > GT_SPARSE * GT_NT * GT_PPT;
+            const bool masked = nys <= 64 && nxs <= GT_MASKW && nxs * nys * ezl > GT_SPARSE * GT_NT * GT_PPT;
             if (masked) {
                 unsigned long long* bm = msk;
                 unsigned long long* need = msk + GT_MASKW;
@@ -1134,12 +1139,12 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 __syncthreads();
             }
             {
-                // Staging in 4-cell segments of the slab's rows (row = staged (x, y), ezp
-                // cells of z): item it -> row it / nseg, segment it % nseg.  Cells past ez
-                // in a row's last segment are padding, never read back.
+                // Staging in 4-cell segments of the slab's rows (row = staged (x, y), ezl
+                // cells of z at LDS pitch ezp): item it -> row it / nseg, segment it % nseg.
+                // Cells past ez in a row's last segment are padding, never read back.
                 const int wz0 = wrap_cell(bz0, ng);
                 // the padded row inside one grid row (uniform per workgroup): no z wrap
-                const bool zin = wz0 + ezp <= ng;
+                const bool zin = wz0 + ezl <= ng;
                 const int nitems = nxs * nys * nseg;
                 // item it -> (row, segment), row -> (x, y): this thread's first items by
                 // division, then stepped by ST items a round; the step's quotients and
@@ -1186,7 +1191,7 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                             } else {
 #pragma unroll
                                 for (int c = 0; c < 4; c++) {
-                                    // ezp <= ng + 4 and wz0 < ng: one subtraction wraps (twice
+                                    // ezl <= ng + 4 and wz0 < ng: one subtraction wraps (twice
                                     // for the padding of a row that spans the whole grid)
                                     int z = wz0 + 4 * sg[u] + c;
                                     z = z >= ng ? z - ng : z;
