@@ -13,6 +13,7 @@
 #include "internal.h"
 
 #include <climits>
+#include <cmath>
 #include <type_traits>
 
 #pragma clang fp contract(off)
@@ -56,6 +57,10 @@ __device__ __forceinline__ long long fixed_weight(float dx, float dy, float dz, 
     const float q = w * (mass * 0x1p-42f);
     if constexpr (SMALL) return (long long)(int)rintf(q);
     else return (long long)__float2ll_rn(q);
+}
+template <bool SMALL> __device__ __forceinline__ auto round_weight(float q) {
+    if constexpr (SMALL) return (unsigned)(int)rintf(q);
+    else return (unsigned long long)__float2ll_rn(q);
 }
 // Window cell type of the arena route: 32-bit when a weight fits (SMALL), else 64-bit.
 template <bool SMALL> using DepW = typename std::conditional<SMALL, unsigned, unsigned long long>::type;
@@ -211,7 +216,9 @@ template <int WM, typename F> __device__ __forceinline__ void dep_slabs(const in
 // Adds the corners of the tile's particles that fall in the slab (sx0.., nx) x (sy0.., ny)
 // x (all z) to win (slab order x, y, z); returns this thread's sum of the added weights.
 // ALL: the slab is the whole window, every corner is inside (no checks).
-template <bool SMALL, bool ALL, int PPT, typename W>
+// M2: mass is a power of two, so (w * mass) * 2^24 is an exact scaling of w and the
+// weight is (hx hy) * (gz mass 2^-42): the same bits with one product fewer per corner.
+template <bool SMALL, bool ALL, int PPT, typename W, bool M2 = false>
 __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRegs<PPT>& R, const int lo[3],
                                                              const int e[3], int sx0, int nx, int sy0, int ny,
                                                              float mass) {
@@ -239,6 +246,8 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
         const float gx[2] = {ONE - ox, ox};
         const float gy[2] = {ONE - oy, oy};
         const float gz[2] = {ONE - oz, oz};
+        const float mq = mass * 0x1p-42f;
+        const float gzm[2] = {gz[0] * mq, gz[1] * mq}; // exact when M2
         const int b0 = __mul24(R.rx[k], Sx) + __mul24(R.ry[k], Sy) + CS * R.rz[k] + c0;
         // one particle's 8 weights sum to about mass * 2^24 < 2^31 when SMALL
         DepW<SMALL> pd = 0;
@@ -257,8 +266,15 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
                 const int y = jy ^ ly;
                 if (!ALL && (by + y < 0 || by + y >= ny)) continue;
                 W* row = (W*)(wb + (b0 + rstep[jx][jy]));
-                const DepW<SMALL> q0 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[0], mass);
-                const DepW<SMALL> q1 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[1], mass);
+                DepW<SMALL> q0, q1;
+                if constexpr (M2) {
+                    const float hxy = hx[jx] * hy[jy];
+                    q0 = round_weight<SMALL>(hxy * gzm[0]);
+                    q1 = round_weight<SMALL>(hxy * gzm[1]);
+                } else {
+                    q0 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[0], mass);
+                    q1 = fixed_weight_w<SMALL>(hx[jx], hy[jy], gz[1], mass);
+                }
                 pd += q0 + q1;
                 atomicAdd(row, (W)q0);
                 atomicAdd(row + 1, (W)q1);
@@ -280,7 +296,7 @@ __device__ __forceinline__ unsigned long long dep_accumulate(W* win, const DepRe
 template <bool SMALL> __device__ __forceinline__ bool dep_one_pass(const int e[3]) {
     return e[0] * e[1] * e[2] <= DEP_LDS / (int)sizeof(DepW<SMALL>);
 }
-template <bool SMALL, int PPT>
+template <bool SMALL, int PPT, bool M2 = false>
 __device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRegs<PPT>& R, const int lo[3],
                                                    const int e[3], float mass, unsigned long long& dep) {
     const int vol = e[0] * e[1] * e[2];
@@ -291,7 +307,7 @@ __device__ __forceinline__ void dep_one_accumulate(DepW<SMALL>* win, const DepRe
         for (int i = threadIdx.x; i < vol; i += DEP_NT) win[i] = 0;
     }
     __syncthreads();
-    dep += dep_accumulate<SMALL, true>(win, R, lo, e, 0, e[0], 0, e[1], mass);
+    dep += dep_accumulate<SMALL, true, PPT, DepW<SMALL>, M2>(win, R, lo, e, 0, e[0], 0, e[1], mass);
 }
 // after a barrier that follows dep_one_accumulate
 // base is a multiple of 4 cells (reservations are): whole 4-cell groups move as 16 bytes.
@@ -333,7 +349,7 @@ __device__ __forceinline__ void dep_one_flush(const DepW<SMALL>* win, const int 
 }
 // WIDE (64-bit LDS window, !SMALL): every cell is stored whole as two 32-bit arena
 // words, cell i of the window at words base + 2i, base + 2i + 1.
-template <bool SMALL, bool WIDE, int PPT>
+template <bool SMALL, bool WIDE, int PPT, bool M2 = false>
 __device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const DepRegs<PPT>& R, const int lo[3],
                                                        const int e[3], float mass, unsigned* __restrict arena,
                                                        unsigned long long base, unsigned long long& dep,
@@ -347,7 +363,7 @@ __device__ __forceinline__ void dep_window_arena_slabs(DepW<SMALL>* win, const D
         const int ez = e[2], svol = nx * ny * ez;
         for (int i = tid; i < svol; i += DEP_NT) win[i] = 0;
         __syncthreads();
-        dep += dep_accumulate<SMALL, false>(win, R, lo, e, sx0, nx, sy0, ny, mass);
+        dep += dep_accumulate<SMALL, false, PPT, W, M2>(win, R, lo, e, sx0, nx, sy0, ny, mass);
         __syncthreads();
         // thread walks cells tid, tid + DEP_NT, ... of the slab with incremental
         // (ix, iy, iz) so no division runs inside the loop; the arena index advances with
@@ -497,7 +513,7 @@ struct DepWindowShared {
 // Only the 32-bit pass stores the gather's boxes.  q: the tile's entry of wide (WIDE);
 // a tile of at most DEP_WKEEP chunks takes the slots it had in the 32-bit pass again
 // (the same chunks, in the same order), a larger one lists its chunks anew.
-template <bool SMALL, bool WIDE, int PPT>
+template <bool SMALL, bool WIDE, int PPT, bool M2 = false>
 __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const DepWindowArgs& g,
                                                  DepWindowShared& sh) {
     constexpr bool S32 = SMALL && !WIDE;           // 32-bit LDS window
@@ -571,7 +587,7 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
     }
     unsigned long long dep = 0, fl = 0;
     const bool one = dep_one_pass<S32>(e);
-    if (one) dep_one_accumulate<S32>(win, R, lo, e, g.mass, dep);
+    if (one) dep_one_accumulate<S32, PPT, M2>(win, R, lo, e, g.mass, dep);
     if (fits) {
         if (tid == 0) {
             const unsigned long long sub_cap = (g.cap / g.nsub) & ~3ull;
@@ -611,7 +627,7 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
         return;
     }
     if (one) dep_one_flush<S32, WIDE>(win, e, g.arena, sh.base, fl);
-    else dep_window_arena_slabs<S32, WIDE>(win, R, lo, e, g.mass, g.arena, sh.base, dep, fl);
+    else dep_window_arena_slabs<S32, WIDE, PPT, M2>(win, R, lo, e, g.mass, g.arena, sh.base, dep, fl);
     // workgroup totals: the stored cells sum to the deposited weights unless a cell wrapped
     unsigned long long dif = dep - fl;
     for (int off = 32; off > 0; off >>= 1) {
@@ -643,10 +659,10 @@ __device__ __forceinline__ void dep_windows_tile(int tile, unsigned q, const Dep
     }
 }
 
-template <bool SMALL, int PPT>
+template <bool SMALL, int PPT, bool M2>
 __global__ __launch_bounds__(DEP_NT) void k_deposit_windows(DepWindowArgs g) {
     __shared__ __align__(16) DepWindowShared sh;
-    dep_windows_tile<SMALL, false, PPT>((int)xcd_remap(blockIdx.x, gridDim.x), 0u, g, sh);
+    dep_windows_tile<SMALL, false, PPT, M2>((int)xcd_remap(blockIdx.x, gridDim.x), 0u, g, sh);
 }
 
 // The tiles the 32-bit pass listed in wide (ctl[2] of them), before the owner pass.
@@ -1525,16 +1541,20 @@ void launch_deposit_windows(const PosPackH* pos, int64_t n, int ng, float mass, 
     if (!n) return;
     const unsigned nt = (unsigned)(n / (DEP_NT * b.ppt));
     const DepWindowArgs g = window_args(pos, n, ng, mass, b);
+    // a power-of-two mass (ng / np a power of two): the weights' product with the mass is exact
+    int ex = 0;
+    const bool m2 = std::frexp(mass, &ex) == 0.5f;
     // the tiles with a cell over 32 bits (none, mostly: the workgroups of the second
     // launch return at once)
     const unsigned grid = nt < 256 ? nt : 256;
     auto run = [&](auto pc) {
         constexpr int PPT = decltype(pc)::value;
         if (mass < 128.0f) {
-            k_deposit_windows<true, PPT><<<nt, DEP_NT, 0, s>>>(g);
+            if (m2) k_deposit_windows<true, PPT, true><<<nt, DEP_NT, 0, s>>>(g);
+            else k_deposit_windows<true, PPT, false><<<nt, DEP_NT, 0, s>>>(g);
             k_deposit_wide<true, PPT><<<grid, DEP_NT, 0, s>>>(g);
         } else {
-            k_deposit_windows<false, PPT><<<nt, DEP_NT, 0, s>>>(g);
+            k_deposit_windows<false, PPT, false><<<nt, DEP_NT, 0, s>>>(g);
             k_deposit_wide<false, PPT><<<grid, DEP_NT, 0, s>>>(g);
         }
     };

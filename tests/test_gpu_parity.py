@@ -56,6 +56,25 @@ def test_deposit_bit_exact(hg, orc, ng, clustered):
         assert np.array_equal(m.get_density(), got)
 
 
+@pytest.mark.parametrize("ng,np_,clustered", [(96, 64, False), (96, 64, True), (64, 32, True)])
+def test_deposit_bit_exact_particle_mass(hg, orc, ng, np_, clustered):
+    """Particle mass (ng/np)^3 other than 1: 3.375 (not a power of two: the weights'
+    product with the mass rounds) and 8 (a power of two: the windows pass folds the
+    mass into the z factors, an exact scaling).  Bit-exact against the oracle."""
+    case(f"deposit mass ng={ng} np={np_} clustered={clustered}")
+    rng = np.random.default_rng(7 * ng + np_ + clustered)
+    gp = np.float32(ng) / np.float32(np_)
+    mass = float(gp * gp * gp)
+    with mk(hg, ng, np_) as m:
+        cell, off = rand_split(rng, m.N, ng, clustered)
+        vel = np.zeros((m.N, 3), np.float32)
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        got = m.get_density()
+        ref = orc.cic_split(ng, cell, off, mass)
+        assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ng,np_", [(64, 64), (128, 64), (256, 256)])
 def test_deposit_routes(hg, orc, ng, np_):
     """Arena route (windows -> owner blocks), atomic route for deferred tiles, and the
