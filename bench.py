@@ -67,8 +67,9 @@ def kernel_bytes(name, Np, G):
 
 def measured_traffic(workload, region):
     """HBM bytes per launch of a timed region, from the committed counter summaries
-    (profiles/*.md lines `traffic: <workload> <region> <bytes>`; FETCH_SIZE and
-    WRITE_SIZE passes scaled by a plain copy of known size).  None if not measured."""
+    (profiles/*.md lines `traffic: <workload>/steps<first>-<last> <region> <bytes>`, the
+    counter passes over the same timed steps; FETCH_SIZE and WRITE_SIZE passes scaled by
+    a plain copy of known size).  None if not measured."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
@@ -552,7 +553,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9 if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK if achieved else None,
-                     "traffic": measured_traffic(workload, name),
+                     "traffic": measured_traffic(f"{workload}/steps{args.warmup}-{args.warmup + args.steps - 1}", name),
                      "alg_bytes_per_launch": per_launch, "avg_launch_ms": avg_ms},
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (ms_step * 1e-3) / 1e9,
                           "frac": step_alg / (ms_step * 1e-3) / HBM_PEAK,
