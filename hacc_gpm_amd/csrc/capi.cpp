@@ -94,8 +94,6 @@ struct hgpm_ctx {
     int var_cols = 2, var_xs = 2;
     int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
     int dep_ppt = 8;    // deposit tile: a whole 16^3 Lagrangian tile (8) or half (4), tunable "deposit_tile"
-    // owner-pass routing: the last count of blocks that needed 64-bit sums the host has
-    // (pinned, copied back after each deposit without waiting) and the current choice
 };
 
 namespace {

@@ -295,7 +295,7 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    only) = those of them that found their slot list or sub-arena full; "deposit_wide"
    (get only) = tiles of the last deposit with a window cell over 32 bits, stored with
    64-bit cells; "deposit_owner_redo" (get only) = owner blocks of the last deposit
-   summed in 64-bit cells (a 64-bit chunk, or a cell sum of 2^32 units or more);
+   that needed 64-bit sums (a 64-bit chunk, or a cell sum of 2^32 units or more);
    "deposit_arena_mwords" (get only) = 32-bit arena words the last
    deposit reserved, in units of 2^20 (every window reservation, those of tiles then
    found full and the doubled ones of the 64-bit pass included).  Unknown keys return HGPM_ERR_ARG. */
