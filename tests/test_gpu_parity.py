@@ -490,7 +490,8 @@ def test_solve_fp64_path(hg, orc, ng):
 
 # --------------------------------------------------------- kick / drift ----
 @pytest.mark.parametrize("ng,np_,dist", [(32, 32, "random"), (64, 64, "random"), (64, 64, "lattice"),
-                                         (128, 64, "lattice"), (64, 64, "sparse"), (512, 32, "lattice")])
+                                         (128, 64, "lattice"), (64, 64, "sparse"), (512, 32, "lattice"),
+                                         (96, 96, "random"), (96, 32, "lattice"), (192, 64, "sparse")])
 def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
     """Kick and drift against the oracle, bit for bit.  "sparse": displacements of 1.5
     cells fill each gather tile's box thinly, so the gather stages only the rows its
@@ -540,6 +541,35 @@ def test_kick_drift_bit_exact(hg, orc, ng, np_, dist):
         c3, o3, _ = m.get_particles_split()
         c3r, o3r = orc.drift_split(ng, cref, oref, vref, -0.4)
         assert np.array_equal(c3, c3r) and np.array_equal(o3, o3r)
+
+
+@pytest.mark.parametrize("ng,np_,dist", [(64, 64, "random"), (96, 32, "random"), (1024, 64, "lattice")])
+def test_fused_drift_large_moves(hg, orc, ng, np_, dist):
+    """The drift inside the gather with moves of up to three boxes, bit-exact against
+    the oracle: moves reduced modulo ng, moves of exactly ng/2, ng and 2 ng, and at
+    ng = 1024 moves past 512 cells, where the power-of-two path's 32-bit sums wrap
+    (ng = 96 takes the general path).  A zero kick leaves the chosen velocities exact."""
+    case(f"fused drift large moves ng={ng} np={np_}")
+    rng = np.random.default_rng(5 * ng + np_)
+    with mk(hg, ng, np_) as m:
+        if dist == "random":
+            cell, off = rand_split(rng, m.N, ng)
+        else:
+            cell, off = lattice_split(rng, np_, ng, 1.5)
+        vel = (rng.uniform(-3.0, 3.0, (m.N, 3)) * ng).astype(np.float32)
+        special = np.array([0.0, ng / 2, -ng / 2, ng - 2.0 ** -10, -(ng - 2.0 ** -10), ng, -ng, 2 * ng,
+                            -3 * ng, 511.99997, 512.0, -512.0, -512.00006, 1023.99994, 2.0 ** -30],
+                           np.float32)
+        vel.reshape(-1)[:special.size] = special
+        vel.reshape(-1)[special.size:2 * special.size] = special[::-1]
+        m.set_particles_split(cell, off, vel)
+        m.cic()
+        m.solve_gradient()
+        m.kick_drift(0.0, 1.0, 1.0)  # v + 0 * S = v for every finite force
+        c2, o2, v2 = m.get_particles_split()
+        cref, oref = orc.drift_split(ng, cell, off, vel, 1.0)
+        assert np.array_equal(v2, vel)
+        assert np.array_equal(c2, cref) and np.array_equal(o2, oref)
 
 
 def test_fp64_kick_drift(hg, orc):
