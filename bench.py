@@ -387,7 +387,7 @@ def run_ensemble(hg, steps, warmup, per_class, classes_alone=True):
     progress("one box at a time")
     t_single = 0.0
     tuned = {}  # per size class: the autotuner's choices on its first box, kept for the others
-    keys = ("fft_cols", "fft_xsolve", "deposit_tile", "gather_tile")
+    keys = ("fft_plan", "fft_cols", "fft_xsolve", "deposit_tile", "gather_tile")
     for b, p in enumerate(ps):
         with hg.MemoryManager(p) as m:
             m.generate_ic(*ics[b])

@@ -490,7 +490,8 @@ class MemoryManager:
         self._c(lib().hgpm_autotune(self.h, int(reps), _ptr(mc), _ptr(mx)))
         out = {"cols_ms": [v if v >= 0 else None for v in mc.tolist()],
                "xsolve_ms": [v if v >= 0 else None for v in mx.tolist()],
-               "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve")}
+               "fft_cols": self.get_tunable("fft_cols"), "fft_xsolve": self.get_tunable("fft_xsolve"),
+               "fft_plan": self.get_tunable("fft_plan")}
         if self.precision == FP32:
             # the particle kernels, on this context's particles when it has them
             md, mg = np.zeros(2), np.zeros(2)

@@ -37,6 +37,9 @@ def _jobs():
     jobs = []
     for n in FFT_SIZES:
         jobs.append((os.path.join(CSRC, "k_fft.hip"), os.path.join(OBJ, f"k_fft_{n}.o"), [f"-DHGPM_FFT_N={n}"]))
+    # the alternative radix order of the 1024-point transforms
+    jobs.append((os.path.join(CSRC, "k_fft.hip"), os.path.join(OBJ, "k_fft_1024_alt.o"),
+                 ["-DHGPM_FFT_N=1024", "-DHGPM_FFT_PLAN_ALT"]))
     for name in ["k_particles.hip", "k_power.hip", "k_ic.hip", "k_fp64.hip", "capi.cpp", "host.cpp", "ensemble.cpp"]:
         src = os.path.join(CSRC, name)
         if os.path.exists(src):

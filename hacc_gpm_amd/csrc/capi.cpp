@@ -92,6 +92,7 @@ struct hgpm_ctx {
     // tunables: column-pass and x-pass variants of the FFT (see k_fft.hip), set by
     // hgpm_set_tunable or hgpm_autotune
     int var_cols = 2, var_xs = 2;
+    int fft_plan = 0; // radix order of the FFT passes: 0 = fft_ops(ng), 1 = fft_ops_alt(ng) (tunable "fft_plan")
     int var_gather = 0; // gather workgroup shape (k_particles.hip), tunable "gather_tile"
     int dep_ppt = 8;    // deposit tile: a whole 16^3 Lagrangian tile (8) or half (4), tunable "deposit_tile"
 };
@@ -201,6 +202,35 @@ int upload_twiddles(hgpm_ctx* c, const FftOps* ops, V** f, V** r) {
         if (rc) return rc;
         HIPCHK(c, hipMemcpy(*r, vr.data(), sizeof(V) * vr.size(), hipMemcpyHostToDevice));
     }
+    return HGPM_OK;
+}
+
+constexpr int NVAR_MAX = 8; // FFT column / x-pass variants per length (FftOps::n_variants <= NVAR_MAX)
+
+// FP32 contexts: the FFT passes switched to radix order `plan` (0 = fft_ops(ng), 1 =
+// fft_ops_alt(ng)); the twiddle tables are rewritten in place (the same number of
+// stages), after the work queued on the stream.  Results change only by rounding.
+int set_fft_plan(hgpm_ctx* c, int plan) {
+    if (c->prec != HGPM_FP32) return fail(c, HGPM_ERR_UNSUPPORTED, "fft_plan needs an FP32 context");
+    const FftOps* ops = plan == 0 ? fft_ops(c->ng) : (plan == 1 ? fft_ops_alt(c->ng) : nullptr);
+    if (!ops) return fail(c, HGPM_ERR_ARG, "fft_plan must be 0, or 1 where a second radix order exists (NG 1024)");
+    if (ops == c->fft) return HGPM_OK;
+    if (ops->stages != c->fft->stages || ops->n != c->fft->n)
+        return fail(c, HGPM_ERR_UNSUPPORTED, "radix orders with different stage counts");
+    const int n = ops->n, S = ops->stages;
+    std::vector<double> tf(2 * (size_t)S * n), tr(2 * (size_t)S * n);
+    ops->twiddles(tf.data(), tr.data());
+    std::vector<float2> vf((size_t)S * n), vr((size_t)S * n);
+    for (size_t i = 0; i < vf.size(); i++) {
+        vf[i].x = (float)tf[2 * i]; vf[i].y = (float)tf[2 * i + 1];
+        vr[i].x = (float)tr[2 * i]; vr[i].y = (float)tr[2 * i + 1];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->twf, vf.data(), sizeof(float2) * vf.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->twr, vr.data(), sizeof(float2) * vr.size(), hipMemcpyHostToDevice));
+    if (c->twf_ic == c->twf) c->fftic = ops; // np == ng: the initial conditions share the tables
+    c->fft = ops;
+    c->fft_plan = plan;
     return HGPM_OK;
 }
 
@@ -1332,6 +1362,7 @@ int hgpm_set_tunable(hgpm_ctx* c, const char* key, int value) {
         c->arena_cap = value < 0 ? c->arena_alloc : std::min<unsigned long long>((unsigned long long)value, c->arena_alloc);
         return HGPM_OK;
     }
+    if (!strcmp(key, "fft_plan")) return set_fft_plan(c, value);
     if (!strcmp(key, "gather_tile")) {
         if (value < 0 || value > 1) return fail(c, HGPM_ERR_ARG, "gather_tile must be 0 or 1");
         c->var_gather = value;
@@ -1364,6 +1395,8 @@ int hgpm_get_tunable(hgpm_ctx* c, const char* key, int* value) {
     }
     if (!strcmp(key, "deposit_slots")) { *value = c->nslots; return HGPM_OK; }
     if (!strcmp(key, "gather_tile")) { *value = c->var_gather; return HGPM_OK; }
+    if (!strcmp(key, "fft_plan")) { *value = c->fft_plan; return HGPM_OK; }
+    if (!strcmp(key, "fft_plans")) { *value = c->prec == HGPM_FP32 && fft_ops_alt(c->ng) ? 2 : 1; return HGPM_OK; }
     if (!strcmp(key, "deposit_tile")) { *value = c->dep_ppt == 8 ? 0 : 1; return HGPM_OK; }
     if (!strcmp(key, "deposit_arena_mwords")) {
         // 32-bit arena words the last deposit reserved, in units of 2^20: the windows'
@@ -1401,46 +1434,70 @@ int hgpm_autotune(hgpm_ctx* c, int reps, double* ms_cols, double* ms_xsolve) {
     if (!c) return HGPM_ERR_ARG;
     if (c->prec != HGPM_FP32) return HGPM_OK; // the FP64 plain path has one shape
     if (reps < 1) reps = 3;
-    const FftOps* f = c->fft;
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
     launch_zero_u64((unsigned long long*)c->A, (size_t)c->G * c->nbox, c->st);
     launch_zero_u64((unsigned long long*)c->B, (size_t)c->G * c->nbox, c->st);
-    double best_c = 1e30, best_x = 1e30;
-    for (int v = 0; v < f->n_variants; v++) {
-        if (!((f->valid_mask >> v) & 1u)) {
-            if (ms_cols) ms_cols[v] = -1.0;
-            if (ms_xsolve) ms_xsolve[v] = -1.0;
-            continue;
+    const bool timing = c->timing;
+    c->timing = false;
+    auto timed = [&](auto&& run, double& per) -> int {
+        run();
+        HIPCHK(c, hipEventRecord(a, c->st));
+        for (int r = 0; r < reps; r++) run();
+        HIPCHK(c, hipEventRecord(b, c->st));
+        HIPCHK(c, hipEventSynchronize(b));
+        LAUNCHCHK(c);
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+        per = ms / reps;
+        return HGPM_OK;
+    };
+    // every radix order (where the length has two): its fastest column and x-pass shapes,
+    // then the whole solve with them; the order with the fastest solve is kept
+    const int nplans = fft_ops_alt(c->ng) ? 2 : 1;
+    double best_solve = 1e30;
+    int best_plan = 0, best_vc = 2, best_vx = 2;
+    std::vector<double> pc((size_t)nplans * NVAR_MAX, -1.0), px((size_t)nplans * NVAR_MAX, -1.0);
+    int rc = HGPM_OK;
+    for (int plan = 0; plan < nplans && !rc; plan++) {
+        if ((rc = set_fft_plan(c, plan))) break;
+        const FftOps* f = c->fft;
+        double best_c = 1e30, best_x = 1e30;
+        int vc = 2, vx = 2;
+        for (int v = 0; v < f->n_variants && v < NVAR_MAX && !rc; v++) {
+            if (!((f->valid_mask >> v) & 1u)) continue;
+            double per = 0.0;
+            if ((rc = timed([&]() { f->pass_half(c->A, 1, -1, c->twf, v, c->nbox, c->st); }, per))) break;
+            pc[(size_t)plan * NVAR_MAX + v] = per;
+            if (per < best_c) { best_c = per; vc = v; }
+            if ((rc = timed([&]() { f->x_psi(c->A, c->twf, c->twr, c->hs, v, c->nbox, c->st); }, per))) break;
+            px[(size_t)plan * NVAR_MAX + v] = per;
+            if (per < best_x) { best_x = per; vx = v; }
         }
-        for (int kind = 0; kind < 2; kind++) {
-            auto run = [&]() {
-                if (kind == 0) f->pass_half(c->A, 1, -1, c->twf, v, c->nbox, c->st);
-                else f->x_psi(c->A, c->twf, c->twr, c->hs, v, c->nbox, c->st);
-            };
-            run();
-            HIPCHK(c, hipEventRecord(a, c->st));
-            for (int r = 0; r < reps; r++) run();
-            HIPCHK(c, hipEventRecord(b, c->st));
-            HIPCHK(c, hipEventSynchronize(b));
-            LAUNCHCHK(c);
-            float ms = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&ms, a, b));
-            double per = ms / reps;
-            if (kind == 0) {
-                if (ms_cols) ms_cols[v] = per;
-                if (per < best_c) { best_c = per; c->var_cols = v; }
-            } else {
-                if (ms_xsolve) ms_xsolve[v] = per;
-                if (per < best_x) { best_x = per; c->var_xs = v; }
-            }
+        if (rc) break;
+        c->var_cols = vc;
+        c->var_xs = vx;
+        c->dens_f32 = false;
+        double per = 0.0;
+        if ((rc = timed([&]() { f32_solve(c); }, per))) break;
+        if (per < best_solve) { best_solve = per; best_plan = plan; best_vc = vc; best_vx = vx; }
+    }
+    if (!rc) rc = set_fft_plan(c, best_plan);
+    c->var_cols = best_vc;
+    c->var_xs = best_vx;
+    c->timing = timing;
+    if (!rc) {
+        const FftOps* f = c->fft;
+        for (int v = 0; v < f->n_variants; v++) {
+            if (ms_cols) ms_cols[v] = v < NVAR_MAX ? pc[(size_t)best_plan * NVAR_MAX + v] : -1.0;
+            if (ms_xsolve) ms_xsolve[v] = v < NVAR_MAX ? px[(size_t)best_plan * NVAR_MAX + v] : -1.0;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     c->have_density = c->have_force = false;
-    return HGPM_OK;
+    return rc;
 }
 
 // Times the deposit tile variants (16^3 and 8 x 16 x 16 Lagrangian blocks per workgroup)

@@ -9,6 +9,11 @@
 #pragma once
 #include "common.h"
 
+// Everything below has internal linkage: k_fft.hip is compiled once per transform length
+// and once more with another radix order for 1024, so Plan<1024> and the host-side
+// twiddle builders differ between objects and must not be merged by the linker.
+namespace {
+
 // ------------------------------------------------------------ plans ----
 template <int N> struct Plan;
 #define HGPM_PLAN(N_, ...)                                                        \
@@ -31,7 +36,11 @@ HGPM_PLAN(256, 16, 16)
 HGPM_PLAN(384, 3, 8, 16)
 HGPM_PLAN(512, 8, 8, 8)
 HGPM_PLAN(768, 3, 16, 16)
+#if defined(HGPM_FFT_PLAN_ALT)
+HGPM_PLAN(1024, 8, 8, 16) // the alternative radix order the autotuner may pick
+#else
 HGPM_PLAN(1024, 16, 8, 8)
+#endif
 #undef HGPM_PLAN
 
 template <int N> __host__ __device__ constexpr int plan_L(int s) {
@@ -281,3 +290,4 @@ template <int N, bool REV> inline void make_twiddles(double* out) {
             }
     }
 }
+} // namespace

@@ -38,6 +38,10 @@ struct FftOps {
 HGPM_FFT_SIZES(HGPM_DECL_OPS)
 #undef HGPM_DECL_OPS
 
+// Transform lengths with a second radix order (the autotuner's "fft_plan" 1).
+extern const FftOps fft_ops_1024_alt;
+inline const FftOps* fft_ops_alt(int n) { return n == 1024 ? &fft_ops_1024_alt : nullptr; }
+
 inline const FftOps* fft_ops(int n) {
     switch (n) {
 #define HGPM_CASE(n) \

@@ -1,5 +1,6 @@
 // k_fft.hip — 3-D FFT passes for one transform length N (compiled once per N with
-// -DHGPM_FFT_N=<N>).  Replaces the cuFFT C2C/Z2Z calls of src/fft_tools/cuda/ffts.cu
+// -DHGPM_FFT_N=<N>, and once more for N = 1024 with -DHGPM_FFT_PLAN_ALT: the same
+// passes over the radix order 8 x 8 x 16 instead of 16 x 8 x 8, fft_ops_1024_alt).  Replaces the cuFFT C2C/Z2Z calls of src/fft_tools/cuda/ffts.cu
 // (plans :60-130, forward :244-294, inverse :397-445: unnormalised, in place,
 // x-major ng^3 grids) and fuses the spectral solve of src/pm_solver/solver.cpp:145-170
 // into the transform passes.
@@ -325,7 +326,11 @@ void twiddles(double* fwd, double* rev) {
 namespace hgpm {
 #define HGPM_CAT2(a, b) a##b
 #define HGPM_CAT(a, b) HGPM_CAT2(a, b)
+#if defined(HGPM_FFT_PLAN_ALT)
+extern const FftOps HGPM_CAT(HGPM_CAT(fft_ops_, HGPM_FFT_N), _alt) = {
+#else
 extern const FftOps HGPM_CAT(fft_ops_, HGPM_FFT_N) = {
+#endif
     N, Plan<N>::S, NVAR, variant_mask(), twiddles, pass_f32, pass_f64, fwd_z_density,
     HZ, pass_half, fwd_z_r2c, x_psi, inv_z_c2r_real,
 };

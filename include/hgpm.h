@@ -287,7 +287,10 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    x-pass shapes (columns per tile x threads: 0 = 4x256, 1 = 4x512, 2 = 8x512,
    3 = 16x512, 4 = 8x1024, 5 = 16x1024, where the tile fits LDS); "fft_variants" (get
    only) is the number of variant ids (HGPM_FFT_VARIANTS) and "fft_variant_mask" (get
-   only) has bit v set when variant v is available for this grid.  Deposit routing (FP32): "deposit_arena" = usable cells of the window
+   only) has bit v set when variant v is available for this grid.  "fft_plan" (FP32)
+   selects the radix order of the transforms where a length has two (NG 1024: 0 =
+   16 x 8 x 8, 1 = 8 x 8 x 16; results differ by rounding only), "fft_plans" (get only)
+   = the number of radix orders for this context.  Deposit routing (FP32): "deposit_arena" = usable cells of the window
    arena (< 0 restores the full allocation of 6 32-bit cells per grid cell), "deposit_slots"
    = chunk descriptors per 16^3 owner block (1..64); tiles that do not fit take the
    atomic route, with identical results.  "deposit_deferred" (get only, synchronises)
@@ -301,11 +304,12 @@ int hgpm_reset_timings(hgpm_ctx* ctx);
    found full and the doubled ones of the 64-bit pass included).  Unknown keys return HGPM_ERR_ARG. */
 int hgpm_set_tunable(hgpm_ctx* ctx, const char* key, int value);
 int hgpm_get_tunable(hgpm_ctx* ctx, const char* key, int* value);
-/* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): times each
-   FFT variant on this context's grids (reps launches each), keeps the fastest, and
-   reports per-variant ms in ms_cols / ms_xsolve (HGPM_FFT_VARIANTS entries each, -1
-   for variants not available on this grid; may be NULL).
-   Clobbers the density and the solve's grids. */
+/* Autotuner (no reference counterpart; HACC_GPM has fixed cuFFT plans): for each radix
+   order of the transforms ("fft_plans"), times each FFT variant on this context's grids
+   (reps launches each) and then the whole solve with the fastest ones; keeps the radix
+   order and variants of the fastest solve and reports that order's per-variant ms in
+   ms_cols / ms_xsolve (HGPM_FFT_VARIANTS entries each, -1 for variants not available
+   on this grid; may be NULL).  Clobbers the density and the solve's grids. */
 int hgpm_autotune(hgpm_ctx* ctx, int reps, double* ms_cols, double* ms_xsolve);
 /* Autotuner of the particle kernels (no reference counterpart), on the context's own
    particles (HGPM_ERR_STATE without them): times the deposit tile shapes (tunable

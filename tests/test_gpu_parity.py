@@ -328,19 +328,21 @@ def test_fft_fp32(hg, ng):
         assert rel_l2(back, ng ** 3 * x.astype(np.complex128)) < 4e-6
 
 
-@pytest.mark.parametrize("ng", [256, 384, 512, 768, 1024])
-def test_fft_fp32_separable(hg, ng):
+@pytest.mark.parametrize("ng,plan", [(256, 0), (384, 0), (512, 0), (768, 0), (1024, 0), (1024, 1)])
+def test_fft_fp32_separable(hg, ng, plan):
     """The larger plans (256 = 16*16, 384 = 3*8*16, 512 = 8*8*8, 768 = 3*16*16,
-    1024 = 16*8*8): a separable grid a(x) b(y) c(z) transforms to A(kx) B(ky) C(kz), with
-    A, B, C the 1-D transforms (numpy, FP64), checked at sampled modes; the inverse
-    returns ng^3 times the grid."""
-    case(f"fft fp32 separable ng={ng}")
+    1024 = 16*8*8 and, "fft_plan" 1, 8*8*16): a separable grid a(x) b(y) c(z)
+    transforms to A(kx) B(ky) C(kz), with A, B, C the 1-D transforms (numpy, FP64),
+    checked at sampled modes; the inverse returns ng^3 times the grid."""
+    case(f"fft fp32 separable ng={ng} plan={plan}")
     rng = np.random.default_rng(ng + 5)
     v = [(rng.standard_normal(ng) + 1j * rng.standard_normal(ng)).astype(np.complex64) for _ in range(3)]
     x = (v[0][:, None] * v[1][None, :])[:, :, None] * v[2][None, None, :]
     V = [np.fft.fft(u.astype(np.complex128)) for u in v]
     idx = rng.integers(0, ng, (200000, 3))
     with mk(hg, ng, 32) as m:
+        if plan:
+            m.set_tunable("fft_plan", plan)
         X = m.fft(x, -1)
         got = X[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.complex128)
         want = V[0][idx[:, 0]] * V[1][idx[:, 1]] * V[2][idx[:, 2]]
@@ -406,15 +408,16 @@ def test_solve_fp32_vs_fp64_path_large(hg, ng):
     assert rel_l2(F.astype(np.float64), F64) < 2e-5
 
 
-@pytest.mark.parametrize("ng", [512, 768, 1024])
-def test_solve_single_mode_closed_form_large(hg, ng):
+@pytest.mark.parametrize("ng,plan,xs", [(512, 0, -1), (768, 0, -1), (1024, 0, -1), (1024, 1, -1), (1024, 1, 5)])
+def test_solve_single_mode_closed_form_large(hg, ng, plan, xs):
     """The FP32 half-spectrum solve at the large plans against the closed form, with no
     other code path in between: rho = 1 + A cos(k.x) gives, under the reference's
     Green's function (calcGreens.hpp:11-26) and central-difference gradient
     (kspace_solver.cu:55-90), F_d = A sin(k_d) sin(k.x) / (2 sum_d (cos k_d - 1)).
     The density goes in as int64 2^-24 units (rounding error 2^-25, relative 1e-7 of A).
-    Relative RMS error over 2^21 sampled cells < 2e-5."""
-    case(f"solve single mode closed form ng={ng}")
+    Relative RMS error over 2^21 sampled cells < 2e-5.  At 1024 also with the second
+    radix order ("fft_plan" 1), with the default x pass and with 16 x 1024."""
+    case(f"solve single mode closed form ng={ng} plan={plan} xs={xs}")
     A = 0.3
     m_ = np.array([ng // 8 + 1, 5, ng // 3])
     j = np.arange(ng, dtype=np.float64)
@@ -424,6 +427,11 @@ def test_solve_single_mode_closed_form_large(hg, ng):
     rho = table[idx]
     del idx
     with mk(hg, ng, 32) as m:
+        if plan:
+            m.set_tunable("fft_plan", plan)
+        if xs >= 0:
+            m.set_tunable("fft_xsolve", xs)
+            m.set_tunable("fft_cols", xs)
         m.set_density(rho)
         del rho
         m.solve_gradient()
@@ -440,6 +448,25 @@ def test_solve_single_mode_closed_form_large(hg, ng):
         err2 += float(np.sum((F[s, d].astype(np.float64) - expect) ** 2))
         ref2 += float(np.sum(expect ** 2))
     assert np.sqrt(err2 / ref2) < 2e-5, np.sqrt(err2 / ref2)
+
+
+def test_fft_plan_tunable(hg):
+    """Radix orders: two at 1024 (the autotuner times both and keeps one), one at 512
+    (selecting a second is refused)."""
+    case("fft plan tunable")
+    with mk(hg, 1024, 32) as m:
+        assert m.get_tunable("fft_plans") == 2
+        m.set_tunable("fft_plan", 1)
+        assert m.get_tunable("fft_plan") == 1
+        m.set_tunable("fft_plan", 0)
+        assert m.get_tunable("fft_plan") == 0
+        m.autotune(1)
+        assert m.get_tunable("fft_plan") in (0, 1)
+    with mk(hg, 512, 32) as m:
+        assert m.get_tunable("fft_plans") == 1
+        with pytest.raises(hg.HgpmError):
+            m.set_tunable("fft_plan", 1)
+        assert m.get_tunable("fft_plan") == 0
 
 
 @pytest.mark.parametrize("ng", [32, 96, 128])
