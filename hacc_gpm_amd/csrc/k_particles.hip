@@ -1105,28 +1105,19 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
     const int ez = b[2] + GT_HALO;   // staged cells per row, <= ng + 3
     const int ezl = (ez + 3) & ~3;   // loaded per row: whole 4-cell segments, <= ng + 4
     const int nseg = ezl >> 2;
-    // Slab shape for LDS row pitch pz: ty base rows (ty + 3 staged) with
-    // 4 (ty + 3) pz <= GMAX, then tx base planes (tx + 3 staged) with
-    // (tx + 3) (ty + 3) pz <= GMAX, both split evenly; pz <= ng + 4 <= 1028 keeps both >= 1.
-    auto shape = [&](int pz, int& sty, int& stx) {
-        sty = min(b[1], GMAX / (4 * pz) - GT_HALO);
-        sty = (b[1] + (b[1] + sty - 1) / sty - 1) / ((b[1] + sty - 1) / sty);
-        stx = min(b[0], GMAX / ((sty + GT_HALO) * pz) - GT_HALO);
-        stx = (b[0] + (b[0] + stx - 1) / stx - 1) / ((b[0] + stx - 1) / stx);
-    };
+    // slab shape: ty base rows (ty + 3 staged) with 4 (ty + 3) ezl <= GMAX, then tx base
+    // planes (tx + 3 staged) with (tx + 3) (ty + 3) ezl <= GMAX, both split evenly;
+    // ezl <= ng + 4 <= 1028 keeps both >= 1
+    int ty = min(b[1], GMAX / (4 * ezl) - GT_HALO);
+    ty = (b[1] + (b[1] + ty - 1) / ty - 1) / ((b[1] + ty - 1) / ty);
+    int tx = min(b[0], GMAX / ((ty + GT_HALO) * ezl) - GT_HALO);
+    tx = (b[0] + (b[0] + tx - 1) / tx - 1) / ((b[0] + tx - 1) / tx);
     // LDS row pitch: the smallest pitch >= ezl that is 16 modulo 32 words, so that the
     // rows a wave's particles read (16 cells of z in each of 4 rows of the lattice
-    // block) start 16 banks apart, when it leaves the slabs as they are with ezl (the
-    // same staged cells); else ezl
+    // block) start 16 banks apart, when the slabs above still fit with it (the same
+    // staged cells); else ezl
     const int p16 = ((ezl + 47) & ~31) - 16;
-    int ty, tx, ezp = p16;
-    shape(ezl, ty, tx);
-    if (p16 != ezl) {
-        int ty16, tx16;
-        shape(p16, ty16, tx16);
-        if (ty16 == ty && tx16 == tx) ezp = p16;
-        else ezp = ezl;
-    }
+    const int ezp = 4 * (ty + GT_HALO) * p16 <= GMAX && (tx + GT_HALO) * (ty + GT_HALO) * p16 <= GMAX ? p16 : ezl;
     for (int sx0 = 0; sx0 < b[0]; sx0 += tx) {
         const int nbx = min(tx, b[0] - sx0), nxs = nbx + GT_HALO;
         for (int sy0 = 0; sy0 < b[1]; sy0 += ty) {
