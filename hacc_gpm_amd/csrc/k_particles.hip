@@ -1001,6 +1001,15 @@ __global__ __launch_bounds__(256) void k_deposit_total(const unsigned long long*
 // force grid, so both give the same bits.
 __device__ __forceinline__ float central_diff(float up, float down) { return up - down; }
 
+// a / b for 0 <= a < 2^20 and 0 < b < 2^20, rb an FP32 reciprocal estimate of b
+// (v_rcp_f32, relative error about 2^-23): the FP32 quotient a rb is within 0.2 of a / b,
+// so its truncation is off by at most one, which one exact remainder test corrects.
+__device__ __forceinline__ int div_rcp(int a, int b, float rb) {
+    const int q = (int)((float)a * rb);
+    const int r = a - __mul24(q, b);
+    return q + (r >= b ? 1 : 0) - (r < 0 ? 1 : 0);
+}
+
 // Gather of the force with the deposit's CIC weights and the kick (ICICKernel,
 // icic_pm.cu:72-127): S = sum_c w_c F_c over the 8 corners in (x,y,z) order, then
 // v += float(deltaT*fscal) S (the reference rounds a factor float(w*deltaT*fscal) per
@@ -1169,13 +1178,14 @@ __global__ __launch_bounds__(GT_NT, GT_NT / 128) void k_gather_slab(PosPack* __r
                 constexpr int ST = GT_SEGS * GT_NT;
                 const int dsg = ST % nseg, drow = ST / nseg, dy = drow % nys, dx = drow / nys;
                 int sg[GT_SEGS], row[GT_SEGS], xs[GT_SEGS], ys[GT_SEGS];
+                const float rseg = __builtin_amdgcn_rcpf((float)nseg), rnys = __builtin_amdgcn_rcpf((float)nys);
 #pragma unroll
                 for (int u = 0; u < GT_SEGS; u++) {
                     const int it = tid + u * GT_NT;
-                    row[u] = it / nseg;
-                    sg[u] = it - row[u] * nseg;
-                    xs[u] = row[u] / nys;
-                    ys[u] = row[u] - xs[u] * nys;
+                    row[u] = div_rcp(it, nseg, rseg);
+                    sg[u] = it - __mul24(row[u], nseg);
+                    xs[u] = div_rcp(row[u], nys, rnys);
+                    ys[u] = row[u] - __mul24(xs[u], nys);
                 }
                 for (int i0 = tid; i0 < nitems; i0 += ST) {
                     float a[GT_SEGS][4];
